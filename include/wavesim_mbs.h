@@ -1,0 +1,142 @@
+/* wavesim_mbs.h -- C ABI of the MI355X (gfx950) modified Born series solver.
+ *
+ * The library replaces the native loop RunWaveSimAlgorithmMex that
+ * WaveSimBase.run_algorithm calls (reference WaveSimBase.m:126-155, call at
+ * WaveSimBase.m:145-146); the MATLAB-branch loop it must reproduce is
+ * WaveSimBase.m:157-197 with the stopping rule of Simulation.m:193-217.
+ *
+ * Arrays are complex single precision in MATLAB column-major order
+ * [y, x, z, pol] (y contiguous), exactly as mxArrays hold them.  Positions and
+ * ROIs are 1-based and inclusive, as in the reference.  Every function returns
+ * WSIM_OK (0) or a negative error code; wsim_last_error() describes the last
+ * failure on the calling thread.  No C++ exception crosses this boundary.
+ */
+#ifndef WAVESIM_MBS_H
+#define WAVESIM_MBS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WSIM_ABI_VERSION 1
+
+enum {
+  WSIM_OK = 0,
+  WSIM_ERR_ARG = -1,          /* invalid argument (shape, null pointer, range) */
+  WSIM_ERR_HIP = -2,          /* HIP runtime error (no device, launch failure) */
+  WSIM_ERR_UNSUPPORTED = -3,  /* a size/shape the kernels do not handle */
+  WSIM_ERR_NOMEM = -4,        /* device allocation failed */
+  WSIM_ERR_ABORTED = -5       /* the callback asked to stop */
+};
+
+typedef struct {
+  float re, im;
+} wsim_cfloat;
+
+/* One wiggle descriptor (WaveSimBase.m:300-316): Fourier coordinates already
+ * shifted by a quarter pixel and divided by sqrt(epsilon), and the real-space
+ * phase ramps.  pxe/gx have Nx entries, pye/gy Ny, pze/gz Nz. */
+typedef struct {
+  const float* pxe;
+  const float* pye;
+  const float* pze;
+  const wsim_cfloat* gx;
+  const wsim_cfloat* gy;
+  const wsim_cfloat* gz;
+} wsim_wiggle;
+
+/* One source patch, i.e. one entry of Source.positions/Source.values
+ * (Source.m:17-21) after Simulation.exec shifted and cropped it to the grid
+ * (Simulation.m:154-156): 1-based [y x z pol] position of its first element,
+ * its extent, and its values in column-major order. */
+typedef struct {
+  double pos[4];
+  int64_t size[4];
+  const wsim_cfloat* values;
+} wsim_source_patch;
+
+/* Progress callback (the role of SimulationCallback.m, invoked by the MEX).
+ * Called between batches of iterations with the diff_energy history so far.
+ * Return nonzero to stop the solve (WSIM_ERR_ABORTED). */
+typedef int (*wsim_callback)(const char* type, int64_t it, const double* diff_energy,
+                             int64_t n, double energy_threshold, void* user);
+
+/* ---------------------------------------------------------------------------
+ * Drop-in for RunWaveSimAlgorithmMex (WaveSimBase.m:145-146):
+ *   [E, diff_energy] = RunWaveSimAlgorithmMex(N, gamma, Nwiggles, wiggles, source,
+ *        epsilon, k02e, roi, maxIter, energy_threshold, callbackString, callback_interval)
+ *  N         [Ny Nx Nz Npol] (Npol 1 = WaveSim, 3 = WaveSimVector)
+ *  gamma     Ny*Nx*Nz complex (WaveSimBase.m:73)
+ *  roi       2x4 column-major doubles: roi[2*c] = start, roi[2*c+1] = end of
+ *            column c in [y x z pol] (Simulation.m:117-124)
+ *  E_out     N[0]*N[1]*N[2]*N[3] complex: the primed field E' = gamma*E on the
+ *            full grid; the caller divides by gamma and crops (WaveSimBase.m:150-153)
+ *  diff_energy_out  one entry per iteration (capacity diff_energy_cap);
+ *            *n_diff_energy receives the number of iterations run.
+ * The loop follows WaveSimBase.m:157-197 exactly: source i/eps/Nw added during
+ * the first Nw iterations, alpha = 0.7, energy refreshed at iterations
+ * 1, 2, 2+interval, ... with interval = min(8, callback_interval)
+ * (Simulation.m:61, 135), stop tested at multiples of Nw. */
+int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles,
+                       const wsim_wiggle* wiggles, int npatches, const wsim_source_patch* patches,
+                       float epsilon, wsim_cfloat k02e, const double roi[8], double max_iter,
+                       double energy_threshold, const char* callback_type, double callback_interval,
+                       wsim_callback cb, void* user, wsim_cfloat* E_out, double* diff_energy_out,
+                       int64_t diff_energy_cap, int64_t* n_diff_energy);
+
+/* ---------------------------------------------------------------------------
+ * Handle API: the same loop with device-resident buffers, for repeated solves
+ * and for nrhs right-hand sides that share gamma (one launch per pass covers
+ * every rhs; each rhs has its own energy history and stop flag).  There is no
+ * reference counterpart beyond the loop above: the reference solves several
+ * sources by calling exec repeatedly (examples/pulse2d.m:31-40). */
+typedef struct wsim_solver wsim_solver;
+
+typedef struct {
+  int64_t N[4];         /* [Ny Nx Nz Npol] */
+  int nrhs;             /* right-hand sides sharing gamma */
+  int nwiggles;
+  float epsilon;
+  wsim_cfloat k02e;
+  int64_t roi[2][4];    /* 1-based inclusive [start; end] of [y x z pol] */
+  int energy_interval;  /* energy_calculation_interval (Simulation.m:61) */
+} wsim_solver_desc;
+
+int wsim_solver_create(wsim_solver** out, const wsim_solver_desc* desc, const wsim_cfloat* gamma,
+                       const wsim_wiggle* wiggles);
+/* Replace the source of one rhs (host patches; the library scales by i/eps/Nw). */
+int wsim_solver_set_source(wsim_solver* s, int rhs, int npatches, const wsim_source_patch* patches);
+/* Run the loop from E' = dE' = 0 for every rhs with a nonzero source.
+ * batch = iterations per launch batch (0 = automatic); the batch is captured
+ * once into a HIP graph and replayed. */
+int wsim_solver_solve(wsim_solver* s, int64_t max_iter, double energy_threshold, int64_t batch,
+                      const char* callback_type, double callback_interval, wsim_callback cb,
+                      void* user);
+/* Per-rhs outcome: iterations run, converged flag, diff_energy history
+ * (up to cap entries) and the every-iteration ROI energy (may be null). */
+int wsim_solver_result(wsim_solver* s, int rhs, int64_t* n_iter, int* converged, double* diff_energy,
+                       double* energy_every, int64_t cap);
+/* Copy E' (primed, full grid, N[0]*N[1]*N[2]*N[3] entries) of one rhs to the host. */
+int wsim_solver_get_field(wsim_solver* s, int rhs, wsim_cfloat* E_out);
+/* Device pointers: which = 0 E', 1 dE', 2 gamma, 3 work field T, 4 scaled source. */
+int wsim_solver_device_ptr(wsim_solver* s, int which, int rhs, void** dptr);
+/* One application of the propagation operator of wiggle w (WaveSim.m:15-36 /
+ * WaveSimVector.m:19-51) to a host field; in and out hold N[0..3] entries. */
+int wsim_solver_propagate(wsim_solver* s, int wiggle, const wsim_cfloat* in, wsim_cfloat* out);
+/* Average duration (ms) of each kernel of one iteration, timed with HIP events
+ * on the solver's stream over `reps` iterations; names[i] gets a static string. */
+int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, double* bytes,
+                        const char** names, int* nkernels);
+int wsim_solver_destroy(wsim_solver* s);
+
+const char* wsim_last_error(void);
+int wsim_abi_version(void);
+/* Host-side self checks of the index arithmetic the kernels rely on. */
+int wsim_selftest(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVESIM_MBS_H */

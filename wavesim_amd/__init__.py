@@ -1,0 +1,14 @@
+"""MI355X-native modified Born series (MBS) solver.
+
+The MBS loop of the reference wavesim (WaveSimBase.run_algorithm /
+RunWaveSimAlgorithmMex) runs on one GPU as hand-written gfx950 HIP kernels in
+``libwavesim_mbs.so`` (C ABI: include/wavesim_mbs.h).  This package is the
+host side: ``WaveSim`` / ``WaveSimVector`` / ``Source`` mirror the reference
+classes, ``MBSSolver`` wraps the handle API.
+"""
+from .simulation import (Medium, SimulationGrid, Source, SimState, WaveSim, WaveSimBase,
+                         WaveSimVector, efficient_size, symrange)
+from .solver import MBSSolver, RhsResult
+
+__all__ = ["Medium", "SimulationGrid", "Source", "SimState", "WaveSim", "WaveSimBase",
+           "WaveSimVector", "efficient_size", "symrange", "MBSSolver", "RhsResult"]

@@ -1,0 +1,77 @@
+"""Build the HIP solver library in-tree: ``wavesim_amd/libwavesim_mbs.so``.
+
+``python -m wavesim_amd.build`` compiles, for gfx950 with hipcc, the host
+runtime ``csrc/mbs.hip`` and one object per kernel variant of
+``csrc/variant.hip`` (in parallel), then links them.  Packed-FP32
+instructions are disabled on the device side
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libwavesim_mbs.so")
+ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
+
+# (contig, mode, prologue, epilogue) -- must match WSIM_FOR_EACH_VARIANT
+VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
+            (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
+COMPONENTS = (1, 3)
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.sep not in c or os.path.exists(c)):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _flags():
+    return ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument",
+            "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + msg)
+    return msg
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), [])]
+    for ct, m, p, e in VARIANTS:
+        for c in COMPONENTS:
+            name = f"v_{ct}_{m}_{p}_{e}_{c}"
+            units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
+                          [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",
+                           f"-DWSIM_EPI={e}", f"-DWSIM_C={c}"]))
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    cmds = [[hipcc()] + _flags() + defs + ["-c", src, "-o", obj] for src, obj, defs in units]
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for cmd, msg in zip(cmds, ex.map(_run, cmds)):
+            if verbose:
+                print(cmd[-1], flush=True)
+                if msg.strip():
+                    print(msg)
+    tmp = LIB + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [u[1] for u in units])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)
+    print(LIB)

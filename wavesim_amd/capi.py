@@ -1,0 +1,146 @@
+"""ctypes binding of ``libwavesim_mbs.so`` (declared in include/wavesim_mbs.h).
+
+The library is required: importing this module raises if it is missing, and
+there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("WSIM_LIB", os.path.join(_HERE, "libwavesim_mbs.so"))
+
+WSIM_OK = 0
+ERRORS = {-1: "WSIM_ERR_ARG", -2: "WSIM_ERR_HIP", -3: "WSIM_ERR_UNSUPPORTED", -4: "WSIM_ERR_NOMEM",
+          -5: "WSIM_ERR_ABORTED"}
+
+
+class WsimError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class cfloat(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+class wiggle_t(C.Structure):
+    _fields_ = [("pxe", C.c_void_p), ("pye", C.c_void_p), ("pze", C.c_void_p),
+                ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p)]
+
+
+class patch_t(C.Structure):
+    _fields_ = [("pos", C.c_double * 4), ("size", C.c_int64 * 4), ("values", C.c_void_p)]
+
+
+class desc_t(C.Structure):
+    _fields_ = [("N", C.c_int64 * 4), ("nrhs", C.c_int), ("nwiggles", C.c_int),
+                ("epsilon", C.c_float), ("k02e", cfloat), ("roi", (C.c_int64 * 4) * 2),
+                ("energy_interval", C.c_int)]
+
+
+CALLBACK = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_int64, C.POINTER(C.c_double), C.c_int64,
+                       C.c_double, C.c_void_p)
+
+SYMBOLS = {
+    "wsim_run_algorithm": (C.c_int, [C.POINTER(C.c_double), C.c_void_p, C.c_int, C.POINTER(wiggle_t),
+                                     C.c_int, C.POINTER(patch_t), C.c_float, cfloat,
+                                     C.POINTER(C.c_double), C.c_double, C.c_double, C.c_char_p,
+                                     C.c_double, CALLBACK, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]),
+    "wsim_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(desc_t), C.c_void_p,
+                                     C.POINTER(wiggle_t)]),
+    "wsim_solver_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(patch_t)]),
+    "wsim_solver_solve": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, C.c_char_p,
+                                    C.c_double, CALLBACK, C.c_void_p]),
+    "wsim_solver_result": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64]),
+    "wsim_solver_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "wsim_solver_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "wsim_solver_propagate": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "wsim_solver_profile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_char_p), C.POINTER(C.c_int)]),
+    "wsim_solver_destroy": (C.c_int, [C.c_void_p]),
+    "wsim_last_error": (C.c_char_p, []),
+    "wsim_abi_version": (C.c_int, []),
+    "wsim_selftest": (C.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the solver library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m wavesim_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc != WSIM_OK:
+        raise WsimError(rc, lib().wsim_last_error().decode(errors="replace"))
+    return rc
+
+
+def c64(a) -> np.ndarray:
+    """complex64, MATLAB column-major (Fortran) order."""
+    return np.asfortranarray(np.asarray(a, dtype=np.complex64))
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
+
+
+def ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Wiggles:
+    """Keeps the host tables of the wiggle descriptors alive for a call."""
+
+    def __init__(self, wiggles):
+        self.keep = []
+        arr = (wiggle_t * len(wiggles))()
+        for i, w in enumerate(wiggles):
+            t = [f32(w["pxe"]), f32(w["pye"]), f32(w["pze"]),
+                 np.ascontiguousarray(np.asarray(w["gx"], np.complex64).ravel()),
+                 np.ascontiguousarray(np.asarray(w["gy"], np.complex64).ravel()),
+                 np.ascontiguousarray(np.asarray(w["gz"], np.complex64).ravel())]
+            self.keep += t
+            arr[i] = wiggle_t(*[ptr(x) for x in t])
+        self.arr = arr
+        self.n = len(wiggles)
+
+
+class Patches:
+    """Source patches: list of (1-based position[4], value array in [y,x,z,pol] order)."""
+
+    def __init__(self, positions, values):
+        self.keep = []
+        n = len(positions)
+        arr = (patch_t * max(n, 1))()
+        for i, (p, v) in enumerate(zip(positions, values)):
+            v = np.asarray(v)
+            v4 = v.reshape(list(v.shape) + [1] * (4 - v.ndim)) if v.ndim < 4 else v
+            vf = c64(v4)
+            self.keep.append(vf)
+            arr[i].pos = (C.c_double * 4)(*[float(x) for x in p])
+            arr[i].size = (C.c_int64 * 4)(*[int(x) for x in v4.shape])
+            arr[i].values = ptr(vf)
+        self.arr = arr
+        self.n = n
+
+
+NULL_CALLBACK = CALLBACK()

@@ -1,0 +1,959 @@
+// Host runtime and C ABI of the MBS solver (see include/wavesim_mbs.h).
+//
+// One wsim_solver owns, on one GPU:
+//   gamma [nvox]            shared by every rhs (WaveSimBase.m:73)
+//   E', dE', T, S           [nrhs][C][nvox] complex64 each
+//   per-wiggle 1-D tables   pe/ramp per axis (WaveSimBase.m:300-316)
+//   twiddle tables          per axis, forward and inverse
+//   DevState                the iteration counter and per-rhs stop state
+// and a list of kernel launches that make one MBS iteration
+// (WaveSimBase.m:165-189): 2D-1 axis passes plus the control kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mbs_config.hpp"
+#include "mbs_kernels.hpp"
+#include "wavesim_mbs.h"
+
+namespace wsim {
+
+static thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      throw Error(e_ == hipErrorOutOfMemory ? WSIM_ERR_NOMEM : WSIM_ERR_HIP,               \
+                  std::string(#x) + ": " + hipGetErrorString(e_) + " (line " +             \
+                      std::to_string(__LINE__) + ")");                                     \
+  } while (0)
+
+static void require(bool ok, const std::string& msg, int code = WSIM_ERR_ARG) {
+  if (!ok) throw Error(code, msg);
+}
+
+// ----------------------------------------------------------------------------
+// index arithmetic
+// ----------------------------------------------------------------------------
+static FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f{1u, 0u, 0u};
+  if (d <= 1) return f;
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;  // ceil(log2 d) >= 1
+  const uint64_t m = ((1ull << (31 + s)) + d - 1) / d;
+  f.d = d;
+  f.m = (uint32_t)m;
+  f.sh = s - 1;
+  return f;
+}
+
+static uint32_t host_fdiv(uint32_t x, const FastDiv& f) {
+  if (f.d == 1) return x;
+  return (uint32_t)(((uint64_t)x * f.m) >> 32) >> f.sh;
+}
+
+// radices for a transform of length n: the power-of-two part split as evenly
+// as possible into radix 2..16 stages, then the odd primes 3, 5, 7, 11
+static std::vector<int> default_radices(int n) {
+  std::vector<int> r;
+  int p2 = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++p2;
+  }
+  if (p2 > 0) {
+    const int k = (p2 + 3) / 4;
+    for (int i = 0; i < k; ++i) {
+      const int bits = p2 / k + (i < p2 % k ? 1 : 0);
+      r.push_back(1 << bits);
+    }
+  }
+  for (int p : {11, 7, 5, 3}) {
+    while (n % p == 0) {
+      n /= p;
+      r.push_back(p);
+    }
+  }
+  std::sort(r.begin(), r.end(), [](int a, int b) {
+    const bool pa = (a & (a - 1)) == 0, pb = (b & (b - 1)) == 0;
+    if (pa != pb) return pa;  // powers of two first
+    return a > b;
+  });
+  if (n != 1) r.clear();  // a prime factor above 11: unsupported
+  return r;
+}
+
+struct AxisTables {
+  AxisPlan plan;
+  std::vector<float2> twf, twi;
+};
+
+static AxisTables build_axis(int n, const std::vector<int>& radices) {
+  AxisTables t{};
+  AxisPlan& p = t.plan;
+  p.n = n;
+  p.nst = (int)radices.size();
+  require(p.nst <= MAXST, "too many FFT stages", WSIM_ERR_UNSUPPORTED);
+  auto fill = [&](Stage* st, std::vector<float2>& tw, const std::vector<int>& rad, int sign) {
+    int Ns = 1;
+    for (size_t s = 0; s < rad.size(); ++s) {
+      const int R = rad[s];
+      st[s].R = R;
+      st[s].Ns = Ns;
+      st[s].nb = n / R;
+      st[s].tw = (int)tw.size();
+      st[s].fNs = make_fastdiv((uint32_t)Ns);
+      st[s].fnb = make_fastdiv((uint32_t)(n / R));
+      for (int k = 0; k < Ns; ++k)
+        for (int r = 1; r < R; ++r) {
+          const double ang = sign * 2.0 * M_PI * (double)r * (double)k / ((double)Ns * R);
+          tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+        }
+      Ns *= R;
+    }
+  };
+  std::vector<int> rev(radices.rbegin(), radices.rend());
+  fill(p.f, t.twf, radices, -1);
+  fill(p.i, t.twi, rev, +1);
+  if (t.twf.empty()) t.twf.push_back(make_float2(1.f, 0.f));
+  if (t.twi.empty()) t.twi.push_back(make_float2(1.f, 0.f));
+  return t;
+}
+
+// One block: reduce the energy partials of every rhs, record diff_energy the
+// way Simulation.next does (Simulation.m:193-217) and advance the iteration.
+__global__ void __launch_bounds__(256) control_kernel(DevState* st, const double* partials, int nblk) {
+  __shared__ double red[4];
+  if (st->all_stopped) return;
+  const int it = st->it;
+  const int nrhs = st->nrhs;
+  int all = 1;
+  for (int r = 0; r < nrhs; ++r) {
+    if (st->stopped[r]) continue;
+    double e = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 256) e += partials[(size_t)r * nblk + i];
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double en = red[0] + red[1] + red[2] + red[3];
+      const size_t h = (size_t)r * st->hist_cap + (size_t)(it - 1);
+      st->energy_all[h] = en;
+      if (st->calc_energy) st->last_energy[r] = en;
+      const double de = st->last_energy[r];
+      st->diff[h] = de;
+      const bool can = (it % st->nw) == 0;
+      const double d1 = st->diff[(size_t)r * st->hist_cap];
+      if (can && de / d1 < st->threshold) {
+        st->stopped[r] = 1;
+        st->converged[r] = 1;
+        st->n_iter[r] = it;
+      } else if ((can && it >= st->max_iter) || it >= st->hist_cap) {
+        st->stopped[r] = 1;
+        st->converged[r] = 0;
+        st->n_iter[r] = it;
+      }
+    }
+    __syncthreads();
+    all &= st->stopped[r];
+  }
+  if (threadIdx.x == 0) {
+    st->calc_energy = ((it - 1) % st->interval) == 0;
+    st->it = it + 1;
+    st->all_stopped = all;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// kernel table
+// ----------------------------------------------------------------------------
+constexpr int NT = kThreads;
+constexpr size_t LDS_MAX = 64 * 1024;
+
+static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C) {
+#define WSIM_SELECT(ct, m, p, e)                                           \
+  if ((contig ? 1 : 0) == ct && mode == m && pro == p && epi == e)         \
+    return C == 1 ? wsim_kptr_##ct##_##m##_##p##_##e##_1()                  \
+                  : wsim_kptr_##ct##_##m##_##p##_##e##_3();
+  WSIM_FOR_EACH_VARIANT(WSIM_SELECT)
+#undef WSIM_SELECT
+  throw Error(WSIM_ERR_UNSUPPORTED, "no kernel for this pass variant");
+}
+
+// LDS in front of the transform data: column table and the per-row ramp and
+// coordinate tables (layout of axis_pass_kernel)
+static size_t lds_header(int ncol, int n) {
+  auto r16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+  return r16(ncol * sizeof(ColInfo)) + r16((size_t)n * sizeof(float2)) + r16((size_t)n * sizeof(float));
+}
+
+static int its_for(int C, int R) { return its_per_stage(ept_for(C), C, R); }
+
+struct Launch {
+  KernelFn fn;
+  dim3 grid;
+  size_t lds;
+  PassArgs args;
+  const char* name;
+  double bytes;  // algorithmic HBM bytes of one launch
+};
+
+// ----------------------------------------------------------------------------
+// the solver
+// ----------------------------------------------------------------------------
+struct Solver {
+  int N[3] = {1, 1, 1};
+  int C = 1, nrhs = 1, nw = 1, interval = 8;
+  long long nvox = 0;
+  float epsilon = 1.f;
+  float2 k02e{0.f, 0.f};
+  int roi_lo[4], roi_hi[4];
+  std::vector<int> axes;  // non-trivial axes in [y x z] order
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  float2 *gamma = nullptr, *E = nullptr, *dE = nullptr, *T = nullptr, *S = nullptr;
+  float2* probe = nullptr;  // scratch for wsim_solver_propagate
+  std::vector<char> has_source;
+  double* partials = nullptr;
+  int nblk = 1;
+  DevState* dst = nullptr;
+  DevState* dprobe = nullptr;
+  int *d_stopped = nullptr, *d_conv = nullptr, *d_niter = nullptr;
+  double *d_last = nullptr, *d_diff = nullptr, *d_eall = nullptr;
+  int *p_stopped = nullptr;  // probe state arrays
+  double* p_dummy = nullptr;
+  int hist_cap = 0;
+  float *d_pe[3] = {nullptr, nullptr, nullptr};
+  float2 *d_ramp[3] = {nullptr, nullptr, nullptr};
+  AxisTables tab[3];
+  float2 *d_twf[3] = {nullptr, nullptr, nullptr}, *d_twi[3] = {nullptr, nullptr, nullptr};
+  std::vector<Launch> iter;  // one MBS iteration
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_iters = 0;
+  DevState host_state{};
+  long long last_iters = 0;
+
+  ~Solver() { release(); }
+
+  void release() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    void* ptrs[] = {gamma, E, dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
+                    d_last, d_diff, d_eall, p_stopped, p_dummy};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    for (int a = 0; a < 3; ++a) {
+      if (d_pe[a]) (void)hipFree(d_pe[a]);
+      if (d_ramp[a]) (void)hipFree(d_ramp[a]);
+      if (d_twf[a]) (void)hipFree(d_twf[a]);
+      if (d_twi[a]) (void)hipFree(d_twi[a]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  long long field_elems() const { return (long long)C * nvox; }
+
+  template <class T_>
+  static void dalloc(T_** p, size_t n) {
+    HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T_)));
+  }
+
+  // pass configuration: columns per workgroup and LDS size
+  void configure(PassArgs& a, bool contig, int axis, int& blocks, size_t& lds) {
+    const AxisPlan& pl = tab[axis].plan;
+    const int n = N[axis];
+    a.axis = axis;
+    a.n = n;
+    a.inner = 1;
+    for (int b = 0; b < axis; ++b) a.inner *= N[b];
+    a.rstride = contig ? 1 : a.inner;
+    a.plan = pl;
+    auto fits = [&](int ncol) {
+      for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.f[s].R;
+        const long long need = ((long long)ncol * (n / R) + NT - 1) / NT;
+        if (need > its_for(C, R)) return false;
+      }
+      size_t bytes;
+      if (contig)
+        bytes = (size_t)C * ncol * (n + 1) * sizeof(float2);
+      else
+        bytes = (size_t)n * (C * ncol + 1) * sizeof(float2);
+      bytes += lds_header(ncol, n);
+      return bytes <= LDS_MAX;
+    };
+    int ncol = 0;
+    if (contig) {
+      const long long nlines = nvox / n;
+      for (int c = 64; c >= 1; --c)
+        if (fits(c)) {
+          ncol = c;
+          break;
+        }
+      require(ncol > 0, "transform length " + std::to_string(n) + " does not fit one workgroup",
+              WSIM_ERR_UNSUPPORTED);
+      ncol = (int)std::min<long long>(ncol, nlines);
+      a.ncol = ncol;
+      a.lds_pitch = n + 1;
+      a.nlines = (int)nlines;
+      blocks = (int)((nlines + ncol - 1) / ncol);
+      a.ntile_inner = 1;
+      a.ftile = make_fastdiv(1);
+    } else {
+      for (int c = 32; c >= 1; c >>= 1)
+        if (fits(c)) {
+          ncol = c;
+          break;
+        }
+      require(ncol > 0, "transform length " + std::to_string(n) + " does not fit one workgroup",
+              WSIM_ERR_UNSUPPORTED);
+      while (ncol > 1 && ncol / 2 >= a.inner) ncol >>= 1;
+      a.ncol = ncol;
+      a.lds_pitch = C * ncol + 1;
+      a.ntile_inner = (a.inner + ncol - 1) / ncol;
+      a.ftile = make_fastdiv((uint32_t)a.ntile_inner);
+      const long long outer = nvox / ((long long)a.inner * n);
+      a.nlines = 0;
+      blocks = (int)(a.ntile_inner * outer);
+    }
+    if (contig)
+      lds = (size_t)C * a.ncol * (n + 1) * sizeof(float2);
+    else
+      lds = (size_t)n * (C * a.ncol + 1) * sizeof(float2);
+    lds += lds_header(a.ncol, n);
+  }
+
+  PassArgs base_args() const {
+    PassArgs a{};
+    for (int d = 0; d < 3; ++d) a.N[d] = N[d];
+    a.nvox = nvox;
+    a.rhs_stride = field_elems();
+    for (int d = 0; d < 3; ++d) {
+      a.pe[d] = d_pe[d];
+      a.ramp[d] = d_ramp[d];
+    }
+    a.k02e = k02e;
+    const double kr = k02e.x, ki = k02e.y, kk = kr * kr + ki * ki;
+    a.inv_k02e = make_float2((float)(kr / kk), (float)(-ki / kk));
+    a.invN = (float)(1.0 / (double)nvox);
+    for (int d = 0; d < 4; ++d) {
+      a.roi_lo[d] = roi_lo[d];
+      a.roi_hi[d] = roi_hi[d];
+    }
+    a.st = dst;
+    return a;
+  }
+
+  Launch make_launch(bool contig, int mode, int pro, int epi, int axis, const char* name) {
+    Launch L{};
+    L.args = base_args();
+    L.args.twf = d_twf[axis];
+    L.args.twi = d_twi[axis];
+    int blocks = 0;
+    configure(L.args, contig, axis, blocks, L.lds);
+    L.fn = select_kernel(contig, mode, pro, epi, C);
+    L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
+    L.name = name;
+    const double perv = (epi == EPI_MIX) ? (40.0 * C + 8.0) : 16.0 * C;
+    L.bytes = perv * (double)nvox * nrhs;
+    if (L.lds > 48 * 1024)
+      HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)L.lds));
+    return L;
+  }
+
+  // the launches of one iteration (in = dE', out = T, T in place, T -> E', dE')
+  void build_iteration(std::vector<Launch>& out, bool probe_mode) {
+    out.clear();
+    const int D = (int)axes.size();
+    const int epi = probe_mode ? EPI_RAMP : EPI_MIX;
+    const float2* in0 = probe_mode ? probe : dE;
+    if (D == 1) {
+      Launch L = make_launch(true, MODE_MID, PRO_SRC_RAMP, epi, axes[0], "pass_mid_pro_epi");
+      L.args.in = in0;
+      L.args.S = probe_mode ? nullptr : S;
+      L.args.out = probe_mode ? probe : T;
+      L.args.dE = dE;
+      L.args.E = E;
+      L.args.gamma = gamma;
+      L.args.partials = partials;
+      out.push_back(L);
+    } else {
+      Launch L0 = make_launch(true, MODE_FWD, PRO_SRC_RAMP, EPI_NONE, axes[0], "pass_fwd_pro");
+      L0.args.in = in0;
+      L0.args.S = probe_mode ? nullptr : S;
+      L0.args.out = T;
+      out.push_back(L0);
+      for (int k = 1; k < D - 1; ++k) {
+        Launch L = make_launch(false, MODE_FWD, PRO_NONE, EPI_NONE, axes[k], "pass_fwd");
+        L.args.in = T;
+        L.args.out = T;
+        out.push_back(L);
+      }
+      Launch LM = make_launch(false, MODE_MID, PRO_NONE, EPI_NONE, axes[D - 1], "pass_mid_green");
+      LM.args.in = T;
+      LM.args.out = T;
+      out.push_back(LM);
+      for (int k = D - 2; k >= 1; --k) {
+        Launch L = make_launch(false, MODE_INV, PRO_NONE, EPI_NONE, axes[k], "pass_inv");
+        L.args.in = T;
+        L.args.out = T;
+        out.push_back(L);
+      }
+      Launch LE = make_launch(true, MODE_INV, PRO_NONE, epi, axes[0], "pass_inv_mix");
+      LE.args.in = T;
+      LE.args.out = probe_mode ? probe : T;
+      LE.args.dE = dE;
+      LE.args.E = E;
+      LE.args.gamma = gamma;
+      LE.args.partials = partials;
+      out.push_back(LE);
+    }
+    if (probe_mode)
+      for (auto& L : out) {
+        L.args.st = dprobe;
+        L.grid.y = 1;
+      }
+  }
+
+  void launch(const Launch& L) {
+    hipLaunchKernelGGL(L.fn, L.grid, dim3(NT), L.lds, stream, L.args);
+    HIPCHK(hipGetLastError());
+  }
+
+  void launch_control() {
+    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(256), 0, stream, dst, partials, nblk);
+    HIPCHK(hipGetLastError());
+  }
+
+  void enqueue_iterations(int k) {
+    for (int i = 0; i < k; ++i) {
+      for (const auto& L : iter) launch(L);
+      launch_control();
+    }
+  }
+
+  void create(const wsim_solver_desc* d, const wsim_cfloat* g, const wsim_wiggle* wig) {
+    require(d != nullptr && g != nullptr && wig != nullptr, "null argument");
+    for (int a = 0; a < 3; ++a) {
+      require(d->N[a] >= 1 && d->N[a] < (1 << 30), "grid size out of range");
+      N[a] = (int)d->N[a];
+    }
+    require(d->N[3] == 1 || d->N[3] == 3, "N(4) must be 1 (scalar) or 3 (vector)");
+    C = (int)d->N[3];
+    require(d->nrhs >= 1 && d->nrhs <= 65535, "nrhs out of range");
+    nrhs = d->nrhs;
+    require(d->nwiggles >= 1 && d->nwiggles <= 8, "Nwiggles must be 1..8");
+    nw = d->nwiggles;
+    interval = std::max(1, d->energy_interval);
+    epsilon = d->epsilon;
+    k02e = make_float2(d->k02e.re, d->k02e.im);
+    nvox = (long long)N[0] * N[1] * N[2];
+    require((long long)C * nvox < (1ll << 31), "grid too large for 32-bit in-field indexing",
+            WSIM_ERR_UNSUPPORTED);
+    for (int c = 0; c < 4; ++c) {
+      const long long lim = c < 3 ? N[c] : C;
+      long long lo = d->roi[0][c], hi = d->roi[1][c];
+      require(lo >= 1 && hi <= lim && lo <= hi, "roi outside the grid");
+      roi_lo[c] = (int)(lo - 1);
+      roi_hi[c] = (int)(hi - 1);
+    }
+    axes.clear();
+    for (int a = 0; a < 3; ++a)
+      if (N[a] > 1) axes.push_back(a);
+    if (axes.empty()) axes.push_back(0);
+    require(!(C == 3 && axes.size() == 1), "vector fields need at least two non-trivial axes",
+            WSIM_ERR_UNSUPPORTED);
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // tables
+    for (int a = 0; a < 3; ++a) {
+      std::vector<int> rad = default_radices(N[a]);
+      require(N[a] == 1 || !rad.empty(),
+              "axis length " + std::to_string(N[a]) + " has a prime factor above 11",
+              WSIM_ERR_UNSUPPORTED);
+      if (N[a] == 1) rad = {};
+      tab[a] = build_axis(N[a], rad);
+      dalloc(&d_twf[a], tab[a].twf.size());
+      dalloc(&d_twi[a], tab[a].twi.size());
+      HIPCHK(hipMemcpy(d_twf[a], tab[a].twf.data(), tab[a].twf.size() * sizeof(float2),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_twi[a], tab[a].twi.data(), tab[a].twi.size() * sizeof(float2),
+                       hipMemcpyHostToDevice));
+      std::vector<float> pe((size_t)nw * N[a]);
+      std::vector<float2> rp((size_t)nw * N[a]);
+      for (int w = 0; w < nw; ++w) {
+        const float* src = a == 0 ? wig[w].pye : (a == 1 ? wig[w].pxe : wig[w].pze);
+        const wsim_cfloat* gr = a == 0 ? wig[w].gy : (a == 1 ? wig[w].gx : wig[w].gz);
+        require(src != nullptr && gr != nullptr, "null wiggle table");
+        for (int i = 0; i < N[a]; ++i) {
+          pe[(size_t)w * N[a] + i] = src[i];
+          rp[(size_t)w * N[a] + i] = make_float2(gr[i].re, gr[i].im);
+        }
+      }
+      dalloc(&d_pe[a], pe.size());
+      dalloc(&d_ramp[a], rp.size());
+      HIPCHK(hipMemcpy(d_pe[a], pe.data(), pe.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_ramp[a], rp.data(), rp.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    // fields
+    const size_t fe = (size_t)field_elems() * nrhs;
+    dalloc(&gamma, nvox);
+    HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
+    dalloc(&E, fe);
+    dalloc(&dE, fe);
+    dalloc(&T, fe);
+    dalloc(&S, fe);
+    HIPCHK(hipMemset(S, 0, fe * sizeof(float2)));
+    has_source.assign(nrhs, 0);
+    // state
+    dalloc(&dst, 1);
+    dalloc(&dprobe, 1);
+    dalloc(&d_stopped, nrhs);
+    dalloc(&d_conv, nrhs);
+    dalloc(&d_niter, nrhs);
+    dalloc(&d_last, nrhs);
+    dalloc(&p_stopped, 1);
+    HIPCHK(hipMemset(p_stopped, 0, sizeof(int)));
+    dalloc(&p_dummy, 1);
+    // partials: sized for the energy pass
+    build_iteration(iter, false);
+    nblk = (int)iter.back().grid.x;
+    dalloc(&partials, (size_t)nblk * nrhs);
+    for (auto& L : iter) L.args.partials = partials;
+    // the probe state: one rhs, never stopped
+    DevState ps{};
+    ps.it = nw + 1;
+    ps.nw = nw;
+    ps.nrhs = 1;
+    ps.stopped = p_stopped;
+    HIPCHK(hipMemcpy(dprobe, &ps, sizeof(ps), hipMemcpyHostToDevice));
+  }
+
+  void set_source(int rhs, int np, const wsim_source_patch* patches) {
+    require(rhs >= 0 && rhs < nrhs, "rhs out of range");
+    require(np >= 0 && (np == 0 || patches != nullptr), "null patches");
+    // dense source, pre-multiplied by A = i/eps/Nw in single precision
+    // (WaveSimBase.m:171-172; Source.m:163-169)
+    const size_t fe = (size_t)field_elems();
+    std::vector<float2> h(fe, make_float2(0.f, 0.f));
+    const float ar = 0.f, ai = (1.0f / epsilon) / (float)nw;  // (1i / eps) / Nw
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+      const wsim_source_patch& sp = patches[p];
+      long long pos[4], sz[4];
+      for (int c = 0; c < 4; ++c) {
+        pos[c] = (long long)std::llround(sp.pos[c]) - 1;
+        sz[c] = sp.size[c];
+        const long long lim = c < 3 ? N[c] : C;
+        require(sz[c] >= 0 && pos[c] >= 0 && pos[c] + sz[c] <= lim, "source patch outside the grid");
+      }
+      require(sp.values != nullptr || sz[0] * sz[1] * sz[2] * sz[3] == 0, "null patch values");
+      size_t k = 0;
+      for (long long q = 0; q < sz[3]; ++q)
+        for (long long z = 0; z < sz[2]; ++z)
+          for (long long x = 0; x < sz[1]; ++x)
+            for (long long y = 0; y < sz[0]; ++y, ++k) {
+              const wsim_cfloat v = sp.values[k];
+              const size_t g = (size_t)(pos[3] + q) * nvox +
+                               (size_t)((pos[2] + z) * N[1] + (pos[1] + x)) * N[0] + (size_t)(pos[0] + y);
+              // A * val with A = (ar + i ai)
+              const float re = ar * v.re - ai * v.im;
+              const float im = ar * v.im + ai * v.re;
+              h[g].x += re;
+              h[g].y += im;
+              if (v.re != 0.f || v.im != 0.f) any = true;
+            }
+    }
+    HIPCHK(hipMemcpy(S + (size_t)rhs * fe, h.data(), fe * sizeof(float2), hipMemcpyHostToDevice));
+    has_source[rhs] = any ? 1 : 0;
+  }
+
+  void ensure_history(int cap) {
+    if (cap <= hist_cap) return;
+    if (d_diff) (void)hipFree(d_diff);
+    if (d_eall) (void)hipFree(d_eall);
+    d_diff = d_eall = nullptr;
+    dalloc(&d_diff, (size_t)cap * nrhs);
+    dalloc(&d_eall, (size_t)cap * nrhs);
+    hist_cap = cap;
+  }
+
+  // force_run: every rhs iterates even without a source (kernel timing)
+  void reset_state(long long max_iter, double thr, bool force_run = false) {
+    DevState h{};
+    h.it = 1;
+    h.calc_energy = 1;
+    h.all_stopped = 0;
+    h.nw = nw;
+    h.nrhs = nrhs;
+    h.interval = interval;
+    h.hist_cap = hist_cap;
+    h.nblk = nblk;
+    h.max_iter = max_iter;
+    h.threshold = thr;
+    h.stopped = d_stopped;
+    h.converged = d_conv;
+    h.n_iter = d_niter;
+    h.last_energy = d_last;
+    h.diff = d_diff;
+    h.energy_all = d_eall;
+    std::vector<int> st(nrhs), zero(nrhs, 0);
+    int all = 1;
+    for (int r = 0; r < nrhs; ++r) {
+      st[r] = (has_source[r] || force_run) ? 0 : 1;  // zero source: nothing to do (Simulation.m:157-161)
+      all &= st[r];
+    }
+    h.all_stopped = all;
+    std::vector<double> inf(nrhs, INFINITY);
+    HIPCHK(hipMemcpyAsync(dst, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_stopped, st.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_conv, zero.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_niter, zero.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_last, inf.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, stream));
+    const size_t fe = (size_t)field_elems() * nrhs;
+    HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
+    HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  void capture_graph(int k) {
+    if (gexec && graph_iters == k) return;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    enqueue_iterations(k);
+    HIPCHK(hipStreamEndCapture(stream, &graph));
+    HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    graph_iters = k;
+  }
+
+  int solve(long long max_iter, double thr, long long batch, const char* cbtype, double cbint,
+            wsim_callback cb, void* user) {
+    require(max_iter >= 1, "max_iter must be >= 1");
+    const long long cap = std::min<long long>(max_iter, 2000000);
+    ensure_history((int)cap + 1);
+    reset_state(max_iter, thr);
+    if (batch <= 0) {
+      // about 2 ms of work per batch, a multiple of Nw
+      const double it_us = 112.0 * (double)nvox * C * nrhs / 5e6;  // ~5 TB/s
+      long long k = (long long)std::ceil(2000.0 / std::max(it_us, 1.0));
+      k = std::max<long long>(nw, std::min<long long>(k, 256));
+      batch = ((k + nw - 1) / nw) * nw;
+    }
+    const bool use_graph = std::getenv("WSIM_NO_GRAPH") == nullptr;
+    if (use_graph) capture_graph((int)batch);
+    long long done = 0;
+    double next_cb = cbint > 0 ? cbint : INFINITY;
+    std::vector<double> hist;
+    int rc = WSIM_OK;
+    for (;;) {
+      HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
+      if (host_state.all_stopped) break;
+      if (use_graph)
+        HIPCHK(hipGraphLaunch(gexec, stream));
+      else
+        enqueue_iterations((int)batch);
+      done += batch;
+      HIPCHK(hipStreamSynchronize(stream));
+      if (cb != nullptr) {
+        HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
+        const long long it_now = host_state.it - 1;
+        if (it_now >= next_cb || host_state.all_stopped) {
+          hist.resize((size_t)it_now);
+          HIPCHK(hipMemcpy(hist.data(), d_diff, hist.size() * sizeof(double), hipMemcpyDeviceToHost));
+          const int stop = cb(cbtype ? cbtype : "NoCallback", it_now, hist.data(), (long long)hist.size(),
+                              thr, user);
+          while (next_cb <= it_now) next_cb += cbint;
+          if (stop) {
+            rc = WSIM_ERR_ABORTED;
+            break;
+          }
+        }
+      }
+    }
+    HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
+    last_iters = host_state.it - 1;
+    return rc;
+  }
+
+  void result(int rhs, int64_t* n_iter, int* conv, double* diff, double* eall, int64_t cap) {
+    require(rhs >= 0 && rhs < nrhs, "rhs out of range");
+    int ni = 0, cv = 0, stp = 0;
+    HIPCHK(hipMemcpy(&ni, d_niter + rhs, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&cv, d_conv + rhs, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&stp, d_stopped + rhs, sizeof(int), hipMemcpyDeviceToHost));
+    long long n = stp ? ni : last_iters;  // aborted solves report the iterations run
+    if (n_iter) *n_iter = n;
+    if (conv) *conv = cv;
+    const long long m = std::min<long long>(n, cap);
+    if (m > 0 && diff)
+      HIPCHK(hipMemcpy(diff, d_diff + (size_t)rhs * hist_cap, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (m > 0 && eall)
+      HIPCHK(hipMemcpy(eall, d_eall + (size_t)rhs * hist_cap, m * sizeof(double), hipMemcpyDeviceToHost));
+  }
+
+  void get_field(int rhs, wsim_cfloat* out) {
+    require(rhs >= 0 && rhs < nrhs && out != nullptr, "bad argument");
+    const size_t fe = (size_t)field_elems();
+    HIPCHK(hipMemcpy(out, E + (size_t)rhs * fe, fe * sizeof(float2), hipMemcpyDeviceToHost));
+  }
+
+  void propagate(int w, const wsim_cfloat* in, wsim_cfloat* out) {
+    require(w >= 0 && w < nw && in && out, "bad argument");
+    const size_t fe = (size_t)field_elems();
+    if (!probe) dalloc(&probe, fe);
+    HIPCHK(hipMemcpy(probe, in, fe * sizeof(float2), hipMemcpyHostToDevice));
+    DevState ps{};
+    ps.it = w + 1 + nw;  // wiggle w, past the source iterations
+    ps.nw = nw;
+    ps.nrhs = 1;
+    ps.stopped = p_stopped;
+    HIPCHK(hipMemcpy(dprobe, &ps, sizeof(ps), hipMemcpyHostToDevice));
+    std::vector<Launch> pl;
+    build_iteration(pl, true);
+    for (const auto& L : pl) launch(L);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(out, probe, fe * sizeof(float2), hipMemcpyDeviceToHost));
+  }
+
+  int profile(int reps, int maxk, double* ms, double* bytes, const char** names, int* nk) {
+    require(reps >= 1, "reps must be >= 1");
+    const int K = (int)iter.size() + 1;
+    if (nk) *nk = K;
+    std::vector<hipEvent_t> ev(2 * K);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    std::vector<double> acc(K, 0.0);
+    // run on a fresh state that never stops
+    ensure_history(reps + 2);
+    reset_state(1ll << 40, -1.0, true);
+    for (int r = 0; r < reps; ++r) {
+      for (int k = 0; k < K; ++k) {
+        HIPCHK(hipEventRecord(ev[2 * k], stream));
+        if (k < K - 1)
+          launch(iter[k]);
+        else
+          launch_control();
+        HIPCHK(hipEventRecord(ev[2 * k + 1], stream));
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int k = 0; k < K; ++k) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+        acc[k] += t;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < K && k < maxk; ++k) {
+      if (ms) ms[k] = acc[k] / reps;
+      if (bytes) bytes[k] = k < K - 1 ? iter[k].bytes : 0.0;
+      if (names) names[k] = k < K - 1 ? iter[k].name : "control";
+    }
+    return WSIM_OK;
+  }
+};
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return WSIM_ERR_HIP;
+  } catch (...) {
+    g_err = "unknown error";
+    return WSIM_ERR_HIP;
+  }
+}
+
+}  // namespace wsim
+
+using namespace wsim;
+
+struct wsim_solver {
+  Solver s;
+};
+
+extern "C" {
+
+const char* wsim_last_error(void) { return g_err.c_str(); }
+int wsim_abi_version(void) { return WSIM_ABI_VERSION; }
+
+int wsim_selftest(void) {
+  return guard([&]() -> int {
+    const uint32_t ds[] = {2, 3, 5, 7, 11, 16, 40, 64, 320, 1000, 1152, 2187, 102400, 32768000};
+    for (uint32_t d : ds) {
+      FastDiv f = make_fastdiv(d);
+      for (uint64_t x = 0; x < (1ull << 31); x += 1 + x / 7)
+        if (host_fdiv((uint32_t)x, f) != (uint32_t)x / d) {
+          g_err = "fastdiv mismatch d=" + std::to_string(d) + " x=" + std::to_string(x);
+          return WSIM_ERR_HIP;
+        }
+      for (uint32_t x = (1u << 31) - 4096; x < (1u << 31); ++x)
+        if (host_fdiv(x, f) != x / d) {
+          g_err = "fastdiv mismatch near 2^31 d=" + std::to_string(d);
+          return WSIM_ERR_HIP;
+        }
+    }
+    const int ns[] = {1, 2, 8, 12, 64, 192, 256, 320, 1100, 1152, 2187, 840, 77};
+    for (int n : ns) {
+      auto r = default_radices(n);
+      long long p = 1;
+      for (int x : r) p *= x;
+      if (n > 1 && p != n) {
+        g_err = "radix plan mismatch for n=" + std::to_string(n);
+        return WSIM_ERR_HIP;
+      }
+    }
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_create(wsim_solver** out, const wsim_solver_desc* desc, const wsim_cfloat* gamma,
+                       const wsim_wiggle* wiggles) {
+  return guard([&]() -> int {
+    require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto* h = new wsim_solver();
+    try {
+      h->s.create(desc, gamma, wiggles);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_source(wsim_solver* s, int rhs, int np, const wsim_source_patch* p) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.set_source(rhs, np, p);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_solve(wsim_solver* s, int64_t max_iter, double thr, int64_t batch, const char* cbtype,
+                      double cbint, wsim_callback cb, void* user) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    return s->s.solve(max_iter, thr, batch, cbtype, cbint, cb, user);
+  });
+}
+
+int wsim_solver_result(wsim_solver* s, int rhs, int64_t* n_iter, int* conv, double* diff, double* eall,
+                       int64_t cap) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.result(rhs, n_iter, conv, diff, eall, cap);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_get_field(wsim_solver* s, int rhs, wsim_cfloat* E_out) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.get_field(rhs, E_out);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_device_ptr(wsim_solver* s, int which, int rhs, void** dptr) {
+  return guard([&]() -> int {
+    require(s != nullptr && dptr != nullptr, "null argument");
+    Solver& v = s->s;
+    require(rhs >= 0 && rhs < v.nrhs, "rhs out of range");
+    const size_t off = (size_t)rhs * v.field_elems();
+    switch (which) {
+      case 0: *dptr = v.E + off; break;
+      case 1: *dptr = v.dE + off; break;
+      case 2: *dptr = v.gamma; break;
+      case 3: *dptr = v.T + off; break;
+      case 4: *dptr = v.S + off; break;
+      default: throw Error(WSIM_ERR_ARG, "which must be 0..4");
+    }
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_propagate(wsim_solver* s, int w, const wsim_cfloat* in, wsim_cfloat* out) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.propagate(w, in, out);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_profile(wsim_solver* s, int reps, int maxk, double* ms, double* bytes, const char** names,
+                        int* nk) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    return s->s.profile(reps, maxk, ms, bytes, names, nk);
+  });
+}
+
+int wsim_solver_destroy(wsim_solver* s) {
+  return guard([&]() -> int {
+    delete s;
+    return WSIM_OK;
+  });
+}
+
+int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles, const wsim_wiggle* wiggles,
+                       int npatches, const wsim_source_patch* patches, float epsilon, wsim_cfloat k02e,
+                       const double roi[8], double max_iter, double energy_threshold, const char* callback_type,
+                       double callback_interval, wsim_callback cb, void* user, wsim_cfloat* E_out,
+                       double* diff_energy_out, int64_t diff_energy_cap, int64_t* n_diff_energy) {
+  return guard([&]() -> int {
+    require(N != nullptr && roi != nullptr && E_out != nullptr, "null argument");
+    wsim_solver_desc d{};
+    for (int c = 0; c < 4; ++c) d.N[c] = (int64_t)std::llround(N[c]);
+    d.nrhs = 1;
+    d.nwiggles = nwiggles;
+    d.epsilon = epsilon;
+    d.k02e = k02e;
+    for (int c = 0; c < 4; ++c) {
+      d.roi[0][c] = (int64_t)std::llround(roi[2 * c]);
+      d.roi[1][c] = (int64_t)std::llround(roi[2 * c + 1]);
+    }
+    // energy_calculation_interval = min(8, callback_interval) (Simulation.m:61, 135)
+    d.energy_interval = (int)std::max(1.0, std::min(8.0, callback_interval > 0 ? callback_interval : 8.0));
+    // maxIter: max_iterations or 100000 (WaveSimBase.m:127-131)
+    long long mi = (std::isfinite(max_iter) && max_iter > 0) ? (long long)max_iter : 100000;
+    Solver sv;
+    sv.create(&d, gamma, wiggles);
+    sv.set_source(0, npatches, patches);
+    const int rc = sv.solve(mi, energy_threshold, 0, callback_type, callback_interval, cb, user);
+    int64_t n = 0;
+    int conv = 0;
+    sv.result(0, &n, &conv, diff_energy_out, nullptr, diff_energy_out ? diff_energy_cap : 0);
+    if (n_diff_energy) *n_diff_energy = n;
+    sv.get_field(0, E_out);
+    return rc;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,570 @@
+// Device side of the MBS solver: one templated "axis pass" kernel that runs a
+// batch of 1-D Stockham FFTs along one grid axis through LDS, with the
+// elementwise work of the MBS iteration fused into its first load / last store:
+//
+//   PRO_SRC_RAMP  (first forward pass): load dE' (+ i/eps/Nw * S during the
+//                 first Nw iterations), multiply by the wiggle ramps gx gy gz
+//                 (Source.m:139-172, WaveSim.m:38-48, private/f_wiggle.m:5)
+//   MODE_MID      (middle pass): forward FFT along the last axis, multiply by
+//                 the Green's function (private/f_g0_scalar.m:6 or
+//                 private/f_g0_vector.m:4-8) and 1/N, inverse FFT along the
+//                 same axis -- the spectrum never leaves registers/LDS
+//   EPI_MIX       (last inverse pass): undo the ramps (WaveSim.m:49-59), mix
+//                 and accumulate (private/f_mix.m:3-4, WaveSimBase.m:179-180),
+//                 and the per-workgroup ROI energy partial sums
+//                 (Simulation.m:265-273)
+//
+// Layout: a field is [rhs][comp][z][x][y] complex64 (MATLAB column-major
+// [y,x,z,pol], y contiguous).  A pass along axis `a` sees each component as
+// [outer][n][inner]: the first non-trivial axis has inner == 1 (CONTIG kernel,
+// a workgroup owns `ncol` whole lines), the others are strided (a workgroup
+// owns a tile of `ncol` consecutive inner indices of one outer index, so each
+// row of the tile is one contiguous 8*ncol-byte segment).
+//
+// Stockham autosort (Govindaraju et al. 2008): stage s with radix R and
+// Ns = product of earlier radices reads x[j + r n/R], twiddles by
+// exp(-+2 pi i r (j mod Ns)/(Ns R)), does a radix-R DFT and writes
+// y[(j - j mod Ns) R + j mod Ns + r Ns].  The first stage reads straight from
+// HBM and the last stage writes straight to HBM; only the stages in between go
+// through LDS.  In the middle pass the inverse transform uses the forward
+// radices in reverse order, so the forward transform's last-stage outputs are
+// exactly the inverse transform's first-stage inputs held by the same thread:
+// the Green's-function multiply happens in registers.
+#pragma once
+#include <stdint.h>
+
+#include "fft_radix.hpp"
+
+namespace wsim {
+
+constexpr int MAXST = 10;  // 3^7 = 2187 needs 7 stages
+
+struct FastDiv {
+  uint32_t d, m, sh;  // q = umulhi(x, m) >> sh  for x < 2^31, d >= 2
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv& f) {
+  return f.d == 1 ? x : (__umulhi(x, f.m) >> f.sh);
+}
+
+struct Stage {
+  int R;   // radix
+  int Ns;  // product of the radices of the earlier stages
+  int nb;  // n / R butterflies per line
+  int tw;  // offset of this stage's twiddles: tw[k*(R-1) + r-1], k < Ns, 1 <= r < R
+  FastDiv fNs, fnb;
+};
+
+struct AxisPlan {
+  int n, nst;
+  Stage f[MAXST];  // forward stages (sign -1)
+  Stage i[MAXST];  // inverse stages (sign +1), radices in reverse order
+};
+
+// Iteration state shared by all kernels of the loop; only control_kernel
+// writes it (WaveSimBase.m:165-189, Simulation.m:193-217).
+struct DevState {
+  int it;           // current iteration, 1-based
+  int calc_energy;  // refresh last_energy at this iteration
+  int all_stopped;
+  int nw;           // number of wiggles
+  int nrhs;
+  int interval;     // energy_calculation_interval
+  int hist_cap;     // capacity of the per-rhs histories
+  int nblk;         // energy partials per rhs
+  long long max_iter;
+  double threshold;
+  int* stopped;        // [nrhs]
+  int* converged;      // [nrhs]
+  int* n_iter;         // [nrhs]
+  double* last_energy; // [nrhs]
+  double* diff;        // [nrhs][hist_cap]  diff_energy as the reference records it
+  double* energy_all;  // [nrhs][hist_cap]  |crop dE'|^2 of every iteration
+};
+
+enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
+enum { PRO_NONE = 0, PRO_SRC_RAMP = 1 };
+enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2 };
+
+struct PassArgs {
+  int N[3];  // Ny Nx Nz
+  long long nvox;
+  long long rhs_stride;  // C * nvox
+  int axis;
+  int n;
+  int inner;      // product of N over the axes before `axis`
+  int rstride;    // element stride along the axis (1 or inner)
+  int ncol;       // columns per component in one workgroup
+  int lds_pitch;  // contig: n + pad ; strided: C*ncol + pad  (in float2)
+  int ntile_inner;
+  FastDiv ftile;  // division by ntile_inner
+  int nlines;     // contig: lines per component (nvox / n)
+  AxisPlan plan;
+  const float2* twf;
+  const float2* twi;
+  const float2* in;
+  float2* out;
+  const float2* S;  // premultiplied dense source (i/eps/Nw * S), or null
+  float2* dE;
+  float2* E;
+  const float2* gamma;
+  double* partials;  // [rhs][gridDim.x]
+  const DevState* st;
+  const float* pe[3];     // [nw][N[a]] scaled Fourier coordinates
+  const float2* ramp[3];  // [nw][N[a]] wiggle phase ramps
+  float2 k02e;
+  float2 inv_k02e;
+  float invN;
+  int roi_lo[4];  // 0-based inclusive, [y x z pol]
+  int roi_hi[4];
+};
+
+// butterflies per thread per stage: EPT registers' worth, at most kMaxIts
+// (more unrolled iterations of the small radices only add register pressure)
+constexpr int kMaxIts = 6;
+__host__ __device__ constexpr int its_per_stage(int ept, int C, int R) {
+  return ept / (C * R) < 1 ? 1 : (ept / (C * R) < kMaxIts ? ept / (C * R) : kMaxIts);
+}
+
+struct ColInfo {
+  uint32_t off;  // element offset of row 0 from the workgroup's base (same for every component)
+  float2 rc;     // product of the other axes' ramps
+  float p[3];    // other axes' coordinates (axis entry unused)
+  float q;       // sum of squares of the other axes' coordinates
+  int valid;
+  int roi_ok;    // other axes inside the ROI
+};
+
+// 32-bit element offsets from a workgroup-uniform base pointer, so the
+// compiler can use SGPR-base + 32-bit VGPR-offset addressing
+__device__ __forceinline__ float2 ldg(const float2* base, uint32_t i) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + (size_t)(i * 8u));
+}
+__device__ __forceinline__ void stg(float2* base, uint32_t i, float2 v) {
+  *reinterpret_cast<float2*>(reinterpret_cast<char*>(base) + (size_t)(i * 8u)) = v;
+}
+
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
+struct Pass {
+  const PassArgs& a;
+  float2* lds;
+  const ColInfo* cols;
+  int tid;
+  bool add_src;
+  uint32_t lg_ncol;
+  const float2* ramp_a;  // this wiggle's ramp along the axis
+  const float* pe_a;
+  const float2* in_b[C];  // workgroup bases, per component
+  float2* out_b[C];
+  const float2* S_b[C];
+  float2* dE_b[C];
+  float2* E_b[C];
+  const float2* gam_b;
+  ColInfo cfix;  // strided passes: the thread's column never changes
+  double eacc;
+
+  __device__ __forceinline__ int lidx(int t, int c, int row) const {
+    return CONTIG ? (c * a.ncol + t) * a.lds_pitch + row : row * a.lds_pitch + c * a.ncol + t;
+  }
+
+  // butterfly b -> (column t, butterfly index j)
+  __device__ __forceinline__ void decode(uint32_t b, const Stage& s, int& t, int& j) const {
+    if (CONTIG) {
+      const uint32_t q = fdiv(b, s.fnb);
+      j = (int)(b - q * (uint32_t)s.nb);
+      t = (int)q;
+    } else {
+      t = (int)(b & (uint32_t)(a.ncol - 1));
+      j = (int)(b >> lg_ncol);
+    }
+  }
+
+  __device__ __forceinline__ const ColInfo& col(int t) const { return CONTIG ? cols[t] : cfix; }
+
+  // ---- element transforms --------------------------------------------------
+  __device__ __forceinline__ float2 load_first(const ColInfo& ci, int c, int row) const {
+    const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+    float2 v = ldg(in_b[c], i);
+    if (PRO == PRO_SRC_RAMP) {
+      if (add_src) v = cadd(v, ldg(S_b[c], i));
+      v = cmul(v, cmul(ramp_a[row], ci.rc));
+    }
+    return v;
+  }
+
+  // scalar Green's function times 1/N (f_g0_scalar.m:6)
+  __device__ __forceinline__ float2 green_scalar(const ColInfo& ci, int row, float2 v) const {
+    const float pa = pe_a[row];
+    const float dr = ci.q + (pa * pa - a.k02e.x);
+    const float di = -a.k02e.y;
+    const float s = a.invN * __builtin_amdgcn_rcpf(dr * dr + di * di);
+    return cmul(v, make_float2(dr * s, -di * s));
+  }
+
+  template <int R>
+  static constexpr int its() { return its_per_stage(EPT, C, R); }
+
+  // ---- stage pieces (C components, radix R) --------------------------------
+  template <int R>
+  __device__ __forceinline__ void stage_load_global(float2* v, const Stage& s) const {
+    constexpr int ITS = its<R>();
+    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      int t = 0, j = 0;
+      bool ok = b < total;
+      if (ok) decode(b, s, t, j);
+      const ColInfo& ci = col(t);
+      ok = ok && ci.valid;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[(c * ITS + it) * R + r] = ok ? load_first(ci, c, j + r * s.nb) : make_float2(0.f, 0.f);
+    }
+  }
+
+  template <int R>
+  __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, const float2* tw) const {
+    constexpr int ITS = its<R>();
+    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      if (b < total) {
+        int t, j;
+        decode(b, s, t, j);
+        const int k = j - (int)fdiv((uint32_t)j, s.fNs) * s.Ns;
+        const float2* twk = tw + s.tw + k * (R - 1);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const float2 x = lds[lidx(t, c, j + r * s.nb)];
+            v[(c * ITS + it) * R + r] = r == 0 ? x : cmul(x, twk[r - 1]);
+          }
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[(c * ITS + it) * R + r] = make_float2(0.f, 0.f);
+      }
+    }
+  }
+
+  template <int R, int DIR>
+  __device__ __forceinline__ void stage_compute(float2* v) const {
+    constexpr int ITS = its<R>();
+#pragma unroll
+    for (int k = 0; k < C * ITS; ++k) dft<R, DIR>(v + k * R);
+  }
+
+  template <int R>
+  __device__ __forceinline__ void stage_store_lds(const float2* v, const Stage& s) const {
+    constexpr int ITS = its<R>();
+    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      if (b < total) {
+        int t, j;
+        decode(b, s, t, j);
+        const int k = j - (int)fdiv((uint32_t)j, s.fNs) * s.Ns;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds[lidx(t, c, base + r * s.Ns)] = v[(c * ITS + it) * R + r];
+      }
+    }
+  }
+
+  // last stage: Ns = n / R, outputs land on rows j + r*nb
+  template <int R>
+  __device__ __forceinline__ void stage_store_global(const float2* v, const Stage& s) {
+    constexpr int ITS = its<R>();
+    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      if (b < total) {
+        int t, j;
+        decode(b, s, t, j);
+        const ColInfo& ci = col(t);
+        if (!ci.valid) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = j + r * s.nb;
+          const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+          if (EPI == EPI_NONE) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
+          } else {
+            const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
+            if (EPI == EPI_RAMP) {
+#pragma unroll
+              for (int c = 0; c < C; ++c) stg(out_b[c], i, cmul(v[(c * ITS + it) * R + r], ramp));
+            } else {
+              // f_mix.m:3-4 with alpha = 0.7 : (1 - a g) dE - (0.7 i g^2) Eprop
+              const float2 gm = ldg(gam_b, i);
+              const float2 c1 = make_float2(1.f - 0.7f * gm.x, -0.7f * gm.y);
+              const float2 g2 = cmul(gm, gm);
+              const float2 c2 = make_float2(-0.7f * g2.y, 0.7f * g2.x);
+              const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
+#pragma unroll
+              for (int c = 0; c < C; ++c) {
+                const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
+                const float2 d0 = ldg(dE_b[c], i);
+                const float2 e0 = ldg(E_b[c], i);
+                const float2 dn = csub(cmul(c1, d0), cmul(c2, ep));
+                stg(dE_b[c], i, dn);
+                stg(E_b[c], i, cadd(e0, dn));
+                if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
+                  eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // Green's function on the forward last-stage outputs (rows j + r*nb)
+  template <int R>
+  __device__ __forceinline__ void stage_green(float2* v, const Stage& s) const {
+    constexpr int ITS = its<R>();
+    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      if (b < total) {
+        int t, j;
+        decode(b, s, t, j);
+        const ColInfo& ci = col(t);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = j + r * s.nb;
+          if (C == 1) {
+            v[it * R + r] = green_scalar(ci, row, v[it * R + r]);
+          } else {
+            // f_g0_vector.m:4-8; component 0 <-> x <-> array axis 1
+            const float pr = pe_a[row];
+            const float py = a.axis == 0 ? pr : ci.p[0];
+            const float px = a.axis == 1 ? pr : ci.p[1];
+            const float pz = a.axis == 2 ? pr : ci.p[2];
+            const float2 fx = v[(0 * ITS + it) * R + r];
+            const float2 fy = v[(1 * ITS + it) * R + r];
+            const float2 fz = v[(2 * ITS + it) * R + r];
+            float2 dv = make_float2(px * fx.x + py * fy.x + pz * fz.x, px * fx.y + py * fy.y + pz * fz.y);
+            dv = cmul(dv, a.inv_k02e);
+            const float dr = px * px + py * py + pz * pz - a.k02e.x;
+            const float di = -a.k02e.y;
+            const float sc = a.invN * __builtin_amdgcn_rcpf(dr * dr + di * di);
+            const float2 g0 = make_float2(dr * sc, -di * sc);
+            v[(0 * ITS + it) * R + r] = cmul(g0, csub(fx, cscale(dv, px)));
+            v[(1 * ITS + it) * R + r] = cmul(g0, csub(fy, cscale(dv, py)));
+            v[(2 * ITS + it) * R + r] = cmul(g0, csub(fz, cscale(dv, pz)));
+          }
+        }
+      }
+    }
+  }
+};
+
+#ifdef WSIM_ONLY_RADIX  // experiments: compile a single radix path
+#define WSIM_RADIX_SWITCH(RV, ...) \
+  { constexpr int R = WSIM_ONLY_RADIX; __VA_ARGS__ }
+#else
+#define WSIM_RADIX_SWITCH(RV, ...)                        \
+  switch (RV) {                                           \
+    case 2: { constexpr int R = 2; __VA_ARGS__ } break;   \
+    case 3: { constexpr int R = 3; __VA_ARGS__ } break;   \
+    case 4: { constexpr int R = 4; __VA_ARGS__ } break;   \
+    case 5: { constexpr int R = 5; __VA_ARGS__ } break;   \
+    case 7: { constexpr int R = 7; __VA_ARGS__ } break;   \
+    case 8: { constexpr int R = 8; __VA_ARGS__ } break;   \
+    case 11: { constexpr int R = 11; __VA_ARGS__ } break; \
+    case 16: { constexpr int R = 16; __VA_ARGS__ } break; \
+    default: break;                                       \
+  }
+#endif
+
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
+__global__ void __launch_bounds__(NT) axis_pass_kernel(PassArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int rhs = blockIdx.y;
+  const DevState* st = a.st;
+  if (st->stopped[rhs]) return;
+
+  // LDS: [ColInfo x ncol][ramp_a x n][pe_a x n][transform data]
+  ColInfo* cols = reinterpret_cast<ColInfo*>(smem);
+  const size_t off_ramp = (a.ncol * sizeof(ColInfo) + 15) & ~(size_t)15;
+  const size_t off_pe = off_ramp + (((size_t)a.n * sizeof(float2) + 15) & ~(size_t)15);
+  const size_t off_data = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
+  float2* ramp_l = reinterpret_cast<float2*>(smem + off_ramp);
+  float* pe_l = reinterpret_cast<float*>(smem + off_pe);
+  float2* lds = reinterpret_cast<float2*>(smem + off_data);
+
+  const int it_cur = st->it;
+  const int nw = st->nw;
+  const int w = (it_cur - 1) % nw;
+
+  // workgroup geometry: voxel base and the decomposition of block index
+  long long vblk;
+  int o = 0, itile = 0;
+  if (CONTIG) {
+    vblk = (long long)blockIdx.x * a.ncol * a.n;
+  } else {
+    o = (int)fdiv(blockIdx.x, a.ftile);
+    itile = (int)blockIdx.x - o * a.ntile_inner;
+    vblk = (long long)o * a.n * a.inner + (long long)itile * a.ncol;
+  }
+
+  for (int r = threadIdx.x; r < a.n; r += NT) {
+    ramp_l[r] = a.ramp[a.axis][(size_t)w * a.n + r];
+    pe_l[r] = a.pe[a.axis][(size_t)w * a.n + r];
+  }
+  // per-column coordinates (axes other than a.axis)
+  for (int t = threadIdx.x; t < a.ncol; t += NT) {
+    ColInfo ci;
+    int crd[3] = {0, 0, 0};
+    int valid;
+    if (CONTIG) {
+      const long long line = (long long)blockIdx.x * a.ncol + t;
+      valid = line < a.nlines;
+      ci.off = (uint32_t)(t * a.n);
+      long long rem = valid ? line : 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if (b > a.axis) { crd[b] = (int)(rem % a.N[b]); rem /= a.N[b]; }
+    } else {
+      const int i = itile * a.ncol + t;
+      valid = i < a.inner;
+      ci.off = (uint32_t)t;
+      int rem = valid ? i : 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if (b < a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
+      rem = o;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
+    }
+    ci.valid = valid;
+    float2 rc = make_float2(1.f, 0.f);
+    float q = 0.f;
+    int roi_ok = 1;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      ci.p[b] = 0.f;
+      if (b == a.axis) continue;
+      const float pb = a.pe[b][(size_t)w * a.N[b] + crd[b]];
+      ci.p[b] = pb;
+      q += pb * pb;
+      rc = cmul(rc, a.ramp[b][(size_t)w * a.N[b] + crd[b]]);
+      roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
+    }
+    ci.rc = rc;
+    ci.q = q;
+    ci.roi_ok = roi_ok;
+    cols[t] = ci;
+  }
+  __syncthreads();
+
+  using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT>;
+  P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw, 0u, ramp_l, pe_l};
+  p.eacc = 0.0;
+  if (!CONTIG) {
+    p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
+    p.cfix = cols[threadIdx.x & (a.ncol - 1)];
+  }
+  const long long rbase = (long long)rhs * a.rhs_stride + vblk;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const long long cb = rbase + (long long)c * a.nvox;
+    p.in_b[c] = a.in + cb;
+    p.out_b[c] = a.out + cb;
+    p.S_b[c] = a.S + cb;
+    p.dE_b[c] = a.dE + cb;
+    p.E_b[c] = a.E + cb;
+  }
+  p.gam_b = a.gamma + vblk;
+
+  const AxisPlan& pl = a.plan;
+  const int S = pl.nst;
+  const bool mid = MODE == MODE_MID;
+  const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
+  const float2* tw = (MODE == MODE_INV) ? a.twi : a.twf;
+  constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
+
+  // ---- first (forward, or inverse for MODE_INV) transform -------------------
+  for (int s = 0; s < S; ++s) {
+    const Stage st_ = fs[s];
+    const bool last = s == S - 1;
+    WSIM_RADIX_SWITCH(st_.R,
+      float2 v[P::template its<R>() * C * R];
+      if (s == 0) {
+        p.template stage_load_global<R>(v, st_);
+      } else {
+        p.template stage_load_lds<R>(v, st_, tw);
+        if (!last || mid) __syncthreads();
+      }
+      p.template stage_compute<R, D0>(v);
+      if (!last) {
+        p.template stage_store_lds<R>(v, st_);
+        __syncthreads();
+      } else if (!mid) {
+        p.template stage_store_global<R>(v, st_);
+      } else {
+        // middle pass: Green's function in registers, then the inverse
+        // transform's first stage (same radix, Ns = 1, no twiddles)
+        p.template stage_green<R>(v, st_);
+        p.template stage_compute<R, 1>(v);
+        if (S == 1) {
+          p.template stage_store_global<R>(v, pl.i[0]);
+        } else {
+          p.template stage_store_lds<R>(v, pl.i[0]);
+          __syncthreads();
+        }
+      }
+    )
+  }
+  // ---- remaining inverse stages of the middle pass --------------------------
+  if (mid) {
+    for (int s = 1; s < S; ++s) {
+      const Stage st_ = pl.i[s];
+      const bool last = s == S - 1;
+      WSIM_RADIX_SWITCH(st_.R,
+        float2 v[P::template its<R>() * C * R];
+        p.template stage_load_lds<R>(v, st_, a.twi);
+        if (!last) __syncthreads();
+        p.template stage_compute<R, 1>(v);
+        if (!last) {
+          p.template stage_store_lds<R>(v, st_);
+          __syncthreads();
+        } else {
+          p.template stage_store_global<R>(v, st_);
+        }
+      )
+    }
+  }
+
+  if (EPI == EPI_MIX) {
+    // block reduction of the ROI energy (fixed order: deterministic)
+    __shared__ double red[NT / 64];
+    double e = p.eacc;
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int i = 0; i < NT / 64; ++i) sum += red[i];
+      a.partials[(size_t)rhs * gridDim.x + blockIdx.x] = sum;
+    }
+  }
+}
+
+}  // namespace wsim

@@ -1,0 +1,136 @@
+"""Handle-level wrapper of the HIP solver (wsim_solver_* in include/wavesim_mbs.h).
+
+``MBSSolver`` owns the device buffers of one problem (gamma, wiggle tables,
+E', dE', the work field and the scaled sources of ``nrhs`` right-hand sides)
+and runs the MBS loop of reference WaveSimBase.m:157-197 on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import capi
+
+
+@dataclass
+class RhsResult:
+    iterations: int
+    converged: bool
+    diff_energy: np.ndarray
+    energy_every: np.ndarray
+
+
+class MBSSolver:
+    def __init__(self, N: Sequence[int], gamma, wiggles, epsilon: float, k02e: complex,
+                 roi, energy_interval: int = 8, nrhs: int = 1):
+        L = capi.lib()
+        N = [int(x) for x in N]
+        if len(N) == 3:
+            N = N + [1]
+        self.N = N
+        self.nrhs = int(nrhs)
+        g = capi.c64(np.asarray(gamma).reshape(N[:3]))
+        self._w = capi.Wiggles(wiggles)
+        d = capi.desc_t()
+        d.N = (C.c_int64 * 4)(*N)
+        d.nrhs = self.nrhs
+        d.nwiggles = self._w.n
+        d.epsilon = float(epsilon)
+        d.k02e = capi.cfloat(float(np.real(k02e)), float(np.imag(k02e)))
+        roi = np.asarray(roi, dtype=np.int64)
+        for r in range(2):
+            for c in range(4):
+                d.roi[r][c] = int(roi[r, c])
+        d.energy_interval = int(energy_interval)
+        h = C.c_void_p()
+        capi.check(L.wsim_solver_create(C.byref(h), C.byref(d), capi.ptr(g), self._w.arr))
+        self._h = h
+        self._lib = L
+        self.nw = self._w.n
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wsim_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- setup --------------------------------------------------------------
+    def set_source(self, rhs: int, positions, values):
+        """positions: 1-based [y,x,z,pol] grid positions; values: arrays in
+        [y,x,z,pol] index order (already shifted and cropped to the grid)."""
+        p = capi.Patches(positions, values)
+        capi.check(self._lib.wsim_solver_set_source(self._h, int(rhs), p.n, p.arr))
+
+    # -- run ------------------------------------------------------------------
+    def solve(self, max_iter: int, energy_threshold: float, batch: int = 0,
+              callback: Optional[Callable] = None, callback_interval: float = 0.0,
+              callback_type: str = "NoCallback"):
+        if callback is not None:
+            def _cb(typ, it, de, n, thr, user):
+                hist = np.ctypeslib.as_array(de, shape=(n,)).copy() if n > 0 else np.zeros(0)
+                return 1 if callback(int(it), hist) else 0
+            cb = capi.CALLBACK(_cb)
+        else:
+            cb = capi.NULL_CALLBACK
+        rc = self._lib.wsim_solver_solve(self._h, int(max_iter), float(energy_threshold), int(batch),
+                                         callback_type.encode(), float(callback_interval), cb, None)
+        if rc == -5:
+            return False
+        capi.check(rc)
+        return True
+
+    def result(self, rhs: int = 0, cap: int = 1 << 22) -> RhsResult:
+        n = C.c_int64()
+        cv = C.c_int()
+        capi.check(self._lib.wsim_solver_result(self._h, int(rhs), C.byref(n), C.byref(cv), None, None, 0))
+        m = int(min(n.value, cap))
+        de = np.zeros(max(m, 1))
+        ea = np.zeros(max(m, 1))
+        capi.check(self._lib.wsim_solver_result(
+            self._h, int(rhs), C.byref(n), C.byref(cv),
+            de.ctypes.data_as(C.POINTER(C.c_double)), ea.ctypes.data_as(C.POINTER(C.c_double)), m))
+        return RhsResult(int(n.value), bool(cv.value), de[:m], ea[:m])
+
+    def field(self, rhs: int = 0) -> np.ndarray:
+        """E' (primed, full grid) as a [y,x,z,pol] complex64 array."""
+        out = np.empty(self.N, dtype=np.complex64, order="F")
+        capi.check(self._lib.wsim_solver_get_field(self._h, int(rhs), capi.ptr(out)))
+        return out
+
+    def device_ptr(self, which: int, rhs: int = 0) -> int:
+        p = C.c_void_p()
+        capi.check(self._lib.wsim_solver_device_ptr(self._h, int(which), int(rhs), C.byref(p)))
+        return int(p.value)
+
+    def propagate(self, wiggle: int, E) -> np.ndarray:
+        """One application of the propagation operator (WaveSim.m:15-36 /
+        WaveSimVector.m:19-51) of wiggle `wiggle` (0-based)."""
+        x = capi.c64(np.asarray(E).reshape(self.N))
+        out = np.empty(self.N, dtype=np.complex64, order="F")
+        capi.check(self._lib.wsim_solver_propagate(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
+        return out
+
+    def profile(self, reps: int = 10):
+        """[(kernel name, mean ms, algorithmic bytes)] for one iteration."""
+        K = 16
+        ms = (C.c_double * K)()
+        by = (C.c_double * K)()
+        nm = (C.c_char_p * K)()
+        nk = C.c_int()
+        capi.check(self._lib.wsim_solver_profile(self._h, int(reps), K, ms, by, nm, C.byref(nk)))
+        return [(nm[i].decode(), ms[i], by[i]) for i in range(min(nk.value, K))]
