@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-residual of the MBS solver on one MI355X.
+
+One step = one complete solve of the 256^3 disordered complex64 scalar
+problem (320^3 grid with the 32-px ARL boundary, point source at the centre)
+from E = 0 to relative residual 1e-3 (energy ratio 1e-6), with every input
+already resident in HBM.  Prints one JSON line with the time-to-residual, the
+per-iteration HBM throughput, the roofline of the dominant kernel (HIP-event
+timed on the solver's stream) and the CPU baseline (the numpy restatement of
+the reference loop on this host's cores, bounded sample, extrapolated).
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "time-to-residual 1e-3 (256^3 c64), learned vs plain; HBM GB/s"
+
+
+def bytes_per_voxel_iteration(ndim: int, C: int) -> int:
+    """Algorithmic HBM bytes per voxel per MBS iteration of the fused pipeline
+    (2D-1 passes; the last reads T, dE', E', gamma and writes dE', E')."""
+    return 16 * C * (2 * ndim - 2) + 40 * C + 8
+
+
+def cpu_baseline(nmap, opt, source, gpu_iterations: int, seconds: float, vector=False):
+    """Times the oracle (complex64 numpy/scipy restatement of the reference
+    MATLAB loop, scipy.fft with every host core) on the same workload for a
+    bounded number of iterations and extrapolates to the GPU's iteration count."""
+    from oracle import mbs
+    import scipy.fft
+    cores = os.cpu_count() or 1
+    o = mbs.make_sim(nmap, opt, vector=vector)
+    src = mbs.prepare_source(o, mbs.Source(list(source.positions), list(source.values)))
+    N = o.N
+    E = np.zeros(N, np.complex64)
+    dE = np.zeros(N, np.complex64)
+    nw = len(o.wiggles)
+    A = np.complex64(1j) / np.float32(o.epsilon) / np.float32(nw)
+    g4 = o.gamma[..., None]
+    times = []
+    it = 1
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        wig = o.wiggles[(it - 1) % nw]
+        Etmp = src.add_to(dE.copy(), A) if it <= nw else dE
+        Etmp = mbs.propagate(o, Etmp, wig, workers=cores)
+        dE = mbs.f_mix(dE, Etmp, g4, True)
+        E = E + dE
+        if it <= 2 or (it - 2) % o.energy_calculation_interval == 0:
+            mbs.energy(mbs.crop(o, dE))
+        times.append(time.perf_counter() - t0)
+        it += 1
+        if (time.perf_counter() - t_start > seconds and len(times) >= 4) or len(times) >= 64:
+            break
+    per_it = float(np.median(times[1:])) if len(times) > 1 else times[0]
+    return dict(value=per_it * gpu_iterations, unit="s", cores=cores, kind="port",
+                sample=f"{len(times)} iterations of the numpy/scipy.fft complex64 restatement "
+                       f"(median {per_it * 1e3:.1f} ms/iteration, {cores} threads) "
+                       f"extrapolated to the GPU's {gpu_iterations} iterations",
+                ms_per_iteration=per_it * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="disordered_3d_256")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default FFT plan when none is cached")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the solver is single-GPU); --gpus {args.gpus} "
+              "is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from wavesim_amd import WaveSim, WaveSimVector, plans
+    from wavesim_amd.workloads import WORKLOADS
+
+    vector = args.workload.startswith("sphere_vector")
+    t_setup = time.perf_counter()
+    nmap, opt, source = WORKLOADS[args.workload]()
+    sim = (WaveSimVector if vector else WaveSim)(nmap, opt)
+    sv = sim.solver(1)
+    tuned = "shipped/cached"
+    if plans.lookup(sim.N, 1, len(sim.wiggles)) is None and not args.no_tune:
+        plan, _ = sim.tune(1, 4)
+        tuned = "autotuned at setup"
+    src = sim._prepare(source)
+    sv.set_source(0, src.positions, src.values)
+    max_it = sim.max_iterations()
+    setup_s = time.perf_counter() - t_setup
+    print(f"# setup {setup_s:.1f}s grid {sim.N} nw {len(sim.wiggles)}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        sv.solve(max_it, sim.energy_threshold)
+    torch.cuda.synchronize()
+    times, iters = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sv.solve(max_it, sim.energy_threshold)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        res = sv.result(0)
+        iters.append(res.iterations)
+    assert len(set(iters)) == 1, f"iteration counts differ between runs: {iters}"
+    t_solve = float(np.mean(times))
+    its = iters[0]
+    nvox = int(np.prod(sim.N[:3]))
+    C = sim.N[3]
+    ndim = sum(1 for n in sim.N[:3] if n > 1)
+    bpv = bytes_per_voxel_iteration(ndim, C)
+    it_per_s = its / t_solve
+    hbm_gbps = bpv * nvox * it_per_s / 1e9
+
+    prof = sv.profile(args.profile_reps)
+    kern = [p for p in prof if p[2] > 0]
+    dom = max(kern, key=lambda p: p[1])
+    achieved = dom[2] / (dom[1] * 1e-3) / 1e9
+    iter_ms = sum(p[1] for p in prof)
+    print("# kernels: " + ", ".join(f"{n} {ms * 1e3:.1f}us {b / ms / 1e6:.0f}GB/s" if b else f"{n} {ms * 1e3:.1f}us"
+                                   for n, ms, b in prof), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nmap, opt, source, its, args.cpu_seconds, vector=vector)
+
+    out = {
+        "metric": METRIC,
+        "value": round(t_solve, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_solve * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: grid {sim.N[0]}x{sim.N[1]}x{sim.N[2]}x{C} "
+                               f"(interior + ARL boundary), Nw={len(sim.wiggles)}, plain MBS from "
+                               f"E=0 to residual 1e-3 (energy ratio {sim.energy_threshold:g})",
+                   "iterations": its, "grid": sim.N, "learned_warm_start": None,
+                   "fft_plan": sv.plan, "fft_plan_source": tuned},
+        "iterations_per_s": round(it_per_s, 2),
+        "voxel_updates_per_s": round(C * nvox * it_per_s, 1),
+        "hbm_gbps_algorithmic": round(hbm_gbps, 1),
+        "iteration_roofline_frac": round(hbm_gbps / HBM_PEAK_GBPS, 4),
+        "bytes_per_voxel_iteration": bpv,
+        "kernels_us": {n: round(ms * 1e3, 2) for n, ms, _ in prof},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None, "bytes_per_launch": dom[2], "launch_us": round(dom[1] * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

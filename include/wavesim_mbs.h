@@ -129,6 +129,15 @@ int wsim_solver_propagate(wsim_solver* s, int wiggle, const wsim_cfloat* in, wsi
  * on the solver's stream over `reps` iterations; names[i] gets a static string. */
 int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, double* bytes,
                         const char** names, int* nkernels);
+/* FFT plan of every non-trivial axis, as text "y:8,8,5/16;x:4,16,5/16;..."
+ * (radices of the Stockham stages in forward order / workgroup columns,
+ * 0 = default).  wsim_solver_tune times whole iterations of this problem for
+ * candidate radix orders and tile widths, axis by axis, keeps the fastest and
+ * reports it (the reference picks FFT-friendly sizes only,
+ * SimulationGrid.m:64-80; the per-size plan choice has no counterpart). */
+int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
+int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);
+int wsim_solver_set_plan(wsim_solver* s, const char* plan);
 int wsim_solver_destroy(wsim_solver* s);
 
 const char* wsim_last_error(void);

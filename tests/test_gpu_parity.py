@@ -188,3 +188,36 @@ def test_custom_roi(gpu):
     opt = dict(opt, roi=[[5, 7], [30, 44]])
     _, _, Eo, so, Eg, sg = _run_both(n, opt, src)
     _check_solve(Eo, so, Eg, sg)
+
+
+def test_plan_roundtrip_and_tuned_parity(gpu):
+    """A tuned FFT plan (radix order, tile width) changes speed, not results."""
+    from oracle import mbs
+    n, opt, src = cases.case_3d_disordered()
+    o, g = _pair(n, opt)
+    sv = g.solver()
+    default = sv.plan
+    plan, ms = sv.tune(2)
+    assert ms > 0 and plan
+    sv.plan = "y:4,7/0;x:16,2/8;z:8,3/16"
+    assert sv.plan == "y:4,7/0;x:16,2/8;z:8,3/16"
+    rng = np.random.default_rng(5)
+    E = (rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)).astype(np.complex64)
+    for p in (default, plan, "y:4,7/0;x:16,2/8;z:8,3/16", "y:2,2,7/4;x:2,4,4/32;z:3,8/8"):
+        sv.plan = p
+        for w in (0, 5):
+            assert rel_l2(g.propagate(E, w), mbs.propagate(o, E, o.wiggles[w])) <= TOL, p
+    Eo, so = mbs.run(o, cases.oracle_source(src))
+    sv.plan = plan
+    Eg, sg = g.exec(cases.gpu_source(src))
+    assert sg.iterations == so.it - 1 and rel_l2(Eg, Eo) <= TOL
+
+
+def test_plan_rejects_bad_text(gpu):
+    from wavesim_amd.capi import WsimError
+    n, opt, _ = cases.case_3d_disordered()
+    _, g = _pair(n, opt)
+    sv = g.solver()
+    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y"):
+        with pytest.raises(WsimError):
+            sv.plan = bad

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -69,6 +70,24 @@ static uint32_t host_fdiv(uint32_t x, const FastDiv& f) {
 // as possible into radix 2..16 stages, then the odd primes 3, 5, 7, 11
 static std::vector<int> default_radices(int n) {
   std::vector<int> r;
+  {
+    // experiments/autotuning: WSIM_RADICES_<n>=r0,r1,... overrides the plan
+    const std::string key = "WSIM_RADICES_" + std::to_string(n);
+    if (const char* e = std::getenv(key.c_str())) {
+      long long p = 1;
+      for (const char* q = e; *q;) {
+        const int x = std::atoi(q);
+        if (x > 1) {
+          r.push_back(x);
+          p *= x;
+        }
+        while (*q && *q != ',') ++q;
+        if (*q == ',') ++q;
+      }
+      if (p == n) return r;
+      r.clear();
+    }
+  }
   int p2 = 0;
   while (n % 2 == 0) {
     n /= 2;
@@ -181,7 +200,7 @@ __global__ void __launch_bounds__(256) control_kernel(DevState* st, const double
 // kernel table
 // ----------------------------------------------------------------------------
 constexpr int NT = kThreads;
-constexpr size_t LDS_MAX = 64 * 1024;
+constexpr size_t LDS_CAP = 80 * 1024;  // two workgroups per 160 KB CU
 
 static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C) {
 #define WSIM_SELECT(ct, m, p, e)                                           \
@@ -195,10 +214,14 @@ static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C) {
 
 // LDS in front of the transform data: column table and the per-row ramp and
 // coordinate tables (layout of axis_pass_kernel)
-static size_t lds_header(int ncol, int n) {
+static size_t lds_header(int ncol, int n, int ntw) {
   auto r16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  return r16(ncol * sizeof(ColInfo)) + r16((size_t)n * sizeof(float2)) + r16((size_t)n * sizeof(float));
+  return r16(ncol * sizeof(ColInfo)) + r16((size_t)n * sizeof(float2)) + r16((size_t)n * sizeof(float)) +
+         (size_t)ntw * sizeof(float2);
 }
+
+// contig LDS line pitch: one padding slot every 8 rows, plus one
+static int contig_pitch(int n) { return n + (n >> 3) + 1; }
 
 static int its_for(int C, int R) { return its_per_stage(ept_for(C), C, R); }
 
@@ -207,6 +230,7 @@ struct Launch {
   dim3 grid;
   size_t lds;
   PassArgs args;
+  const PassArgs* dargs;  // device copy of args
   const char* name;
   double bytes;  // algorithmic HBM bytes of one launch
 };
@@ -240,7 +264,13 @@ struct Solver {
   float2 *d_ramp[3] = {nullptr, nullptr, nullptr};
   AxisTables tab[3];
   float2 *d_twf[3] = {nullptr, nullptr, nullptr}, *d_twi[3] = {nullptr, nullptr, nullptr};
+  std::vector<int> plan_rad[3];    // per-axis radix plan (empty: default)
+  int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
   std::vector<Launch> iter;  // one MBS iteration
+  PassArgs* d_args = nullptr;  // device copies of iter[k].args
+  std::vector<Launch> probe_iter;
+  PassArgs* d_probe_args = nullptr;
+  size_t lds_limit = 64 * 1024;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
@@ -255,7 +285,7 @@ struct Solver {
     gexec = nullptr;
     graph = nullptr;
     void* ptrs[] = {gamma, E, dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
-                    d_last, d_diff, d_eall, p_stopped, p_dummy};
+                    d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (int a = 0; a < 3; ++a) {
@@ -276,9 +306,12 @@ struct Solver {
   }
 
   // pass configuration: columns per workgroup and LDS size
-  void configure(PassArgs& a, bool contig, int axis, int& blocks, size_t& lds) {
+  void configure(PassArgs& a, bool contig, int mode, int axis, int& blocks, size_t& lds) {
     const AxisPlan& pl = tab[axis].plan;
     const int n = N[axis];
+    a.ntwf = (int)tab[axis].twf.size();
+    a.ntwi = (int)tab[axis].twi.size();
+    const int ntw = (mode == MODE_INV ? 0 : a.ntwf) + (mode == MODE_FWD ? 0 : a.ntwi);
     a.axis = axis;
     a.n = n;
     a.inner = 1;
@@ -293,16 +326,19 @@ struct Solver {
       }
       size_t bytes;
       if (contig)
-        bytes = (size_t)C * ncol * (n + 1) * sizeof(float2);
+        bytes = (size_t)C * ncol * contig_pitch(n) * sizeof(float2);
       else
         bytes = (size_t)n * (C * ncol + 1) * sizeof(float2);
-      bytes += lds_header(ncol, n);
-      return bytes <= LDS_MAX;
+      bytes += lds_header(ncol, n, ntw);
+      return bytes <= lds_limit;
     };
     int ncol = 0;
+    const char* env_c = std::getenv(contig ? "WSIM_NCOL_CONTIG" : "WSIM_NCOL_STRIDED");
+    int cap = env_c ? std::max(1, std::atoi(env_c)) : (contig ? 16 : 32);
+    if (plan_ncol[axis] > 0) cap = plan_ncol[axis];
     if (contig) {
       const long long nlines = nvox / n;
-      for (int c = 64; c >= 1; --c)
+      for (int c = cap; c >= 1; --c)
         if (fits(c)) {
           ncol = c;
           break;
@@ -311,13 +347,15 @@ struct Solver {
               WSIM_ERR_UNSUPPORTED);
       ncol = (int)std::min<long long>(ncol, nlines);
       a.ncol = ncol;
-      a.lds_pitch = n + 1;
+      a.lds_pitch = contig_pitch(n);
       a.nlines = (int)nlines;
       blocks = (int)((nlines + ncol - 1) / ncol);
       a.ntile_inner = 1;
       a.ftile = make_fastdiv(1);
     } else {
-      for (int c = 32; c >= 1; c >>= 1)
+      int c0 = 1;
+      while (c0 * 2 <= cap) c0 *= 2;
+      for (int c = c0; c >= 1; c >>= 1)
         if (fits(c)) {
           ncol = c;
           break;
@@ -334,10 +372,10 @@ struct Solver {
       blocks = (int)(a.ntile_inner * outer);
     }
     if (contig)
-      lds = (size_t)C * a.ncol * (n + 1) * sizeof(float2);
+      lds = (size_t)C * a.ncol * contig_pitch(n) * sizeof(float2);
     else
       lds = (size_t)n * (C * a.ncol + 1) * sizeof(float2);
-    lds += lds_header(a.ncol, n);
+    lds += lds_header(a.ncol, n, ntw);
   }
 
   PassArgs base_args() const {
@@ -367,7 +405,7 @@ struct Solver {
     L.args.twf = d_twf[axis];
     L.args.twi = d_twi[axis];
     int blocks = 0;
-    configure(L.args, contig, axis, blocks, L.lds);
+    configure(L.args, contig, mode, axis, blocks, L.lds);
     L.fn = select_kernel(contig, mode, pro, epi, C);
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.name = name;
@@ -434,8 +472,195 @@ struct Solver {
   }
 
   void launch(const Launch& L) {
-    hipLaunchKernelGGL(L.fn, L.grid, dim3(NT), L.lds, stream, L.args);
+    hipLaunchKernelGGL(L.fn, L.grid, dim3(NT), L.lds, stream, L.dargs);
     HIPCHK(hipGetLastError());
+  }
+
+  // twiddle tables of axis a for its current radix plan
+  void build_tables(int a) {
+    std::vector<int> rad = plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a];
+    if (N[a] == 1) rad = {};
+    tab[a] = build_axis(N[a], rad);
+    if (d_twf[a]) (void)hipFree(d_twf[a]);
+    if (d_twi[a]) (void)hipFree(d_twi[a]);
+    d_twf[a] = d_twi[a] = nullptr;
+    dalloc(&d_twf[a], tab[a].twf.size());
+    dalloc(&d_twi[a], tab[a].twi.size());
+    HIPCHK(hipMemcpy(d_twf[a], tab[a].twf.data(), tab[a].twf.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_twi[a], tab[a].twi.data(), tab[a].twi.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+
+  // (re)build the launches of one iteration; partials are sized for the
+  // energy pass; cached probe launches and the captured graph are dropped
+  void rebuild_launches() {
+    std::vector<Launch> it2;
+    build_iteration(it2, false);
+    const int nb = (int)it2.back().grid.x;
+    if (!partials || nb > nblk) {
+      if (partials) (void)hipFree(partials);
+      partials = nullptr;
+      dalloc(&partials, (size_t)nb * nrhs);
+    }
+    nblk = nb;
+    for (auto& L : it2) L.args.partials = partials;
+    if (d_args) (void)hipFree(d_args);
+    d_args = nullptr;
+    iter.swap(it2);
+    upload_args(iter, &d_args);
+    probe_iter.clear();
+    if (d_probe_args) (void)hipFree(d_probe_args);
+    d_probe_args = nullptr;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_iters = 0;
+  }
+
+  // mean time (ms) of one steady-state iteration (every rhs forced to run)
+  double time_iteration(int reps) {
+    ensure_history(reps + nw + 2);
+    reset_state(1ll << 40, -1.0, true);
+    enqueue_iterations(nw);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, stream));
+    enqueue_iterations(reps);
+    HIPCHK(hipEventRecord(e1, stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+  }
+
+  // plan text: "y:8,8,5/16;x:4,16,5/16;z:4,16,5/32" (axis:radices/columns)
+  std::string plan_string() const {
+    static const char* nm = "yxz";
+    std::string out;
+    for (int a = 0; a < 3; ++a) {
+      if (N[a] == 1) continue;
+      if (!out.empty()) out += ";";
+      out += nm[a];
+      out += ":";
+      const AxisPlan& p = tab[a].plan;
+      for (int s2 = 0; s2 < p.nst; ++s2) out += (s2 ? "," : "") + std::to_string(p.f[s2].R);
+      out += "/" + std::to_string(plan_ncol[a]);
+    }
+    return out;
+  }
+
+  void set_plan(const std::string& txt) {
+    size_t pos = 0;
+    while (pos < txt.size()) {
+      size_t end = txt.find(';', pos);
+      if (end == std::string::npos) end = txt.size();
+      const std::string item = txt.substr(pos, end - pos);
+      pos = end + 1;
+      if (item.size() < 3 || item[1] != ':') throw Error(WSIM_ERR_ARG, "bad plan item '" + item + "'");
+      const int a = item[0] == 'y' ? 0 : item[0] == 'x' ? 1 : item[0] == 'z' ? 2 : -1;
+      require(a >= 0, "bad plan axis in '" + item + "'");
+      const size_t slash = item.find('/');
+      const std::string rs = item.substr(2, slash == std::string::npos ? std::string::npos : slash - 2);
+      std::vector<int> rad;
+      long long prod = 1;
+      for (size_t q = 0; q < rs.size();) {
+        const int x = std::atoi(rs.c_str() + q);
+        require(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 11 || x == 16,
+                "unsupported radix in plan '" + item + "'");
+        rad.push_back(x);
+        prod *= x;
+        while (q < rs.size() && rs[q] != ',') ++q;
+        ++q;
+      }
+      require(prod == N[a], "plan radices do not multiply to the axis length in '" + item + "'");
+      plan_rad[a] = rad;
+      plan_ncol[a] = slash == std::string::npos ? 0 : std::max(0, std::atoi(item.c_str() + slash + 1));
+      build_tables(a);
+    }
+    rebuild_launches();
+  }
+
+  // every factorisation of n into supported radices with at most `extra`
+  // stages more than the fewest, in every distinct order (capped)
+  static std::vector<std::vector<int>> radix_candidates(int n, int extra, size_t cap) {
+    static const int R[] = {16, 8, 4, 2, 11, 7, 5, 3};
+    std::vector<std::vector<int>> sets;
+    std::vector<int> cur;
+    std::function<void(int, int)> rec = [&](int rem, int start) {
+      if (rem == 1) {
+        sets.push_back(cur);
+        return;
+      }
+      if (cur.size() >= (size_t)MAXST) return;
+      for (int i = start; i < 8; ++i)
+        if (rem % R[i] == 0) {
+          cur.push_back(R[i]);
+          rec(rem / R[i], i);
+          cur.pop_back();
+        }
+    };
+    rec(n, 0);
+    size_t best = 99;
+    for (auto& v : sets) best = std::min(best, v.size());
+    std::vector<std::vector<int>> out;
+    for (auto& v : sets) {
+      if (v.size() > best + extra) continue;
+      std::sort(v.begin(), v.end());
+      size_t k = 0;
+      do {
+        out.push_back(v);
+      } while (++k < 24 && std::next_permutation(v.begin(), v.end()));
+    }
+    if (out.size() > cap) out.resize(cap);
+    return out;
+  }
+
+  // coordinate search over (radix order, workgroup columns) per axis, timing
+  // whole iterations; keeps the fastest plan
+  std::string tune(int reps) {
+    double best = time_iteration(reps);
+    for (int a : axes) {
+      if (N[a] == 1) continue;
+      const bool contig = (a == axes[0]);
+      auto try_cfg = [&](const std::vector<int>& rad, int ncol) {
+        const std::vector<int> old_r = plan_rad[a];
+        const int old_c = plan_ncol[a];
+        plan_rad[a] = rad;
+        plan_ncol[a] = ncol;
+        double t = 1e30;
+        try {
+          build_tables(a);
+          rebuild_launches();
+          t = time_iteration(reps);
+        } catch (const Error&) {
+          t = 1e30;  // the plan does not fit a workgroup
+        }
+        if (t < best) {
+          best = t;
+        } else {
+          plan_rad[a] = old_r;
+          plan_ncol[a] = old_c;
+        }
+      };
+      for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a]);
+      const std::vector<int> cand = contig ? std::vector<int>{4, 8, 12, 16, 24, 32, 48}
+                                           : std::vector<int>{8, 16, 32};
+      for (int c : cand) try_cfg(plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a], c);
+      build_tables(a);
+    }
+    rebuild_launches();
+    return plan_string();
+  }
+
+  void upload_args(std::vector<Launch>& ls, PassArgs** dptr) {
+    std::vector<PassArgs> h(ls.size());
+    for (size_t k = 0; k < ls.size(); ++k) h[k] = ls[k].args;
+    dalloc(dptr, h.size());
+    HIPCHK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(PassArgs), hipMemcpyHostToDevice));
+    for (size_t k = 0; k < ls.size(); ++k) ls[k].dargs = *dptr + k;
   }
 
   void launch_control() {
@@ -482,21 +707,19 @@ struct Solver {
     require(!(C == 3 && axes.size() == 1), "vector fields need at least two non-trivial axes",
             WSIM_ERR_UNSUPPORTED);
     HIPCHK(hipGetDevice(&dev));
+    {
+      int lim = 0;
+      HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+      const char* env_l = std::getenv("WSIM_LDS_CAP_KB");
+      lds_limit = std::min<size_t>((size_t)lim, env_l ? (size_t)std::atoi(env_l) * 1024 : LDS_CAP);
+    }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // tables
     for (int a = 0; a < 3; ++a) {
-      std::vector<int> rad = default_radices(N[a]);
-      require(N[a] == 1 || !rad.empty(),
+      require(N[a] == 1 || !default_radices(N[a]).empty(),
               "axis length " + std::to_string(N[a]) + " has a prime factor above 11",
               WSIM_ERR_UNSUPPORTED);
-      if (N[a] == 1) rad = {};
-      tab[a] = build_axis(N[a], rad);
-      dalloc(&d_twf[a], tab[a].twf.size());
-      dalloc(&d_twi[a], tab[a].twi.size());
-      HIPCHK(hipMemcpy(d_twf[a], tab[a].twf.data(), tab[a].twf.size() * sizeof(float2),
-                       hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_twi[a], tab[a].twi.data(), tab[a].twi.size() * sizeof(float2),
-                       hipMemcpyHostToDevice));
+      build_tables(a);
       std::vector<float> pe((size_t)nw * N[a]);
       std::vector<float2> rp((size_t)nw * N[a]);
       for (int w = 0; w < nw; ++w) {
@@ -533,11 +756,7 @@ struct Solver {
     dalloc(&p_stopped, 1);
     HIPCHK(hipMemset(p_stopped, 0, sizeof(int)));
     dalloc(&p_dummy, 1);
-    // partials: sized for the energy pass
-    build_iteration(iter, false);
-    nblk = (int)iter.back().grid.x;
-    dalloc(&partials, (size_t)nblk * nrhs);
-    for (auto& L : iter) L.args.partials = partials;
+    rebuild_launches();
     // the probe state: one rhs, never stopped
     DevState ps{};
     ps.it = nw + 1;
@@ -729,9 +948,11 @@ struct Solver {
     ps.nrhs = 1;
     ps.stopped = p_stopped;
     HIPCHK(hipMemcpy(dprobe, &ps, sizeof(ps), hipMemcpyHostToDevice));
-    std::vector<Launch> pl;
-    build_iteration(pl, true);
-    for (const auto& L : pl) launch(L);
+    if (probe_iter.empty()) {
+      build_iteration(probe_iter, true);
+      upload_args(probe_iter, &d_probe_args);
+    }
+    for (const auto& L : probe_iter) launch(L);
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(out, probe, fe * sizeof(float2), hipMemcpyDeviceToHost));
   }
@@ -744,8 +965,10 @@ struct Solver {
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     std::vector<double> acc(K, 0.0);
     // run on a fresh state that never stops
-    ensure_history(reps + 2);
+    ensure_history(reps + nw + 2);
     reset_state(1ll << 40, -1.0, true);
+    enqueue_iterations(nw);  // past the source iterations: steady-state kernels
+    HIPCHK(hipStreamSynchronize(stream));
     for (int r = 0; r < reps; ++r) {
       for (int k = 0; k < K; ++k) {
         HIPCHK(hipEventRecord(ev[2 * k], stream));
@@ -912,6 +1135,37 @@ int wsim_solver_profile(wsim_solver* s, int reps, int maxk, double* ms, double* 
   return guard([&]() -> int {
     require(s != nullptr, "null solver");
     return s->s.profile(reps, maxk, ms, bytes, names, nk);
+  });
+}
+
+int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration) {
+  return guard([&]() -> int {
+    require(s != nullptr && reps >= 1, "bad argument");
+    const std::string p = s->s.tune(reps);
+    if (ms_per_iteration) *ms_per_iteration = s->s.time_iteration(reps);
+    if (plan_out && plan_cap > 0) {
+      std::strncpy(plan_out, p.c_str(), (size_t)plan_cap - 1);
+      plan_out[plan_cap - 1] = 0;
+    }
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap) {
+  return guard([&]() -> int {
+    require(s != nullptr && plan_out != nullptr && plan_cap > 0, "bad argument");
+    const std::string p = s->s.plan_string();
+    std::strncpy(plan_out, p.c_str(), (size_t)plan_cap - 1);
+    plan_out[plan_cap - 1] = 0;
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_plan(wsim_solver* s, const char* plan) {
+  return guard([&]() -> int {
+    require(s != nullptr && plan != nullptr, "bad argument");
+    s->s.set_plan(plan);
+    return WSIM_OK;
   });
 }
 

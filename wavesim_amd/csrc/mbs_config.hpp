@@ -9,7 +9,7 @@ constexpr int kThreads = 256;  // threads per workgroup of every axis pass
 // 3-component vector field
 constexpr int ept_for(int C) { return C == 1 ? 32 : 24; }
 
-using KernelFn = void (*)(PassArgs);
+using KernelFn = void (*)(const PassArgs*);
 
 // the pass variants the solver launches, as numbers (contig, mode, prologue,
 // epilogue) with mode 0 fwd / 1 inv / 2 mid, prologue 1 = source + ramps,

@@ -99,6 +99,7 @@ struct PassArgs {
   int ntile_inner;
   FastDiv ftile;  // division by ntile_inner
   int nlines;     // contig: lines per component (nvox / n)
+  int ntwf, ntwi; // twiddle table lengths (float2)
   AxisPlan plan;
   const float2* twf;
   const float2* twi;
@@ -163,8 +164,11 @@ struct Pass {
   ColInfo cfix;  // strided passes: the thread's column never changes
   double eacc;
 
+  // contig lines get one padding slot every 8 rows: the Stockham stores of
+  // the early stages (Ns < 8) write rows R apart, which would otherwise land
+  // on the same LDS banks
   __device__ __forceinline__ int lidx(int t, int c, int row) const {
-    return CONTIG ? (c * a.ncol + t) * a.lds_pitch + row : row * a.lds_pitch + c * a.ncol + t;
+    return CONTIG ? (c * a.ncol + t) * a.lds_pitch + row + (row >> 3) : row * a.lds_pitch + c * a.ncol + t;
   }
 
   // butterfly b -> (column t, butterfly index j)
@@ -392,7 +396,11 @@ struct Pass {
 #endif
 
 template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
-__global__ void __launch_bounds__(NT) axis_pass_kernel(PassArgs a) {
+__global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restrict__ ap) {
+  // the arguments live in device memory (read through the scalar cache):
+  // indexing the stage table of a by-value kernel argument would copy the
+  // whole struct to scratch
+  const PassArgs& a = *ap;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rhs = blockIdx.y;
   const DevState* st = a.st;
@@ -402,10 +410,18 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(PassArgs a) {
   ColInfo* cols = reinterpret_cast<ColInfo*>(smem);
   const size_t off_ramp = (a.ncol * sizeof(ColInfo) + 15) & ~(size_t)15;
   const size_t off_pe = off_ramp + (((size_t)a.n * sizeof(float2) + 15) & ~(size_t)15);
-  const size_t off_data = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
+  const int ntf = MODE == MODE_INV ? 0 : a.ntwf;  // forward twiddles used by this pass
+  const int nti = MODE == MODE_FWD ? 0 : a.ntwi;  // inverse twiddles used by this pass
+  const size_t off_twf = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
+  const size_t off_twi = off_twf + (size_t)ntf * sizeof(float2);
+  const size_t off_data = off_twi + (size_t)nti * sizeof(float2);
   float2* ramp_l = reinterpret_cast<float2*>(smem + off_ramp);
   float* pe_l = reinterpret_cast<float*>(smem + off_pe);
+  float2* twf_l = reinterpret_cast<float2*>(smem + off_twf);
+  float2* twi_l = reinterpret_cast<float2*>(smem + off_twi);
   float2* lds = reinterpret_cast<float2*>(smem + off_data);
+  for (int i = threadIdx.x; i < ntf; i += NT) twf_l[i] = a.twf[i];
+  for (int i = threadIdx.x; i < nti; i += NT) twi_l[i] = a.twi[i];
 
   const int it_cur = st->it;
   const int nw = st->nw;
@@ -432,13 +448,13 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(PassArgs a) {
     int crd[3] = {0, 0, 0};
     int valid;
     if (CONTIG) {
-      const long long line = (long long)blockIdx.x * a.ncol + t;
+      const int line = (int)blockIdx.x * a.ncol + t;  // < nvox < 2^31
       valid = line < a.nlines;
       ci.off = (uint32_t)(t * a.n);
-      long long rem = valid ? line : 0;
+      int rem = valid ? line : 0;
 #pragma unroll
       for (int b = 0; b < 3; ++b)
-        if (b > a.axis) { crd[b] = (int)(rem % a.N[b]); rem /= a.N[b]; }
+        if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
     } else {
       const int i = itile * a.ncol + t;
       valid = i < a.inner;
@@ -494,59 +510,88 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(PassArgs a) {
 
   const AxisPlan& pl = a.plan;
   const int S = pl.nst;
-  const bool mid = MODE == MODE_MID;
+  constexpr bool MID = MODE == MODE_MID;
   const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
-  const float2* tw = (MODE == MODE_INV) ? a.twi : a.twf;
+  const float2* tw = (MODE == MODE_INV) ? twi_l : twf_l;
   constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
 
-  // ---- first (forward, or inverse for MODE_INV) transform -------------------
-  for (int s = 0; s < S; ++s) {
-    const Stage st_ = fs[s];
-    const bool last = s == S - 1;
-    WSIM_RADIX_SWITCH(st_.R,
+  // Each region below is its own radix switch, so the register allocator
+  // sees one short straight-line path per radix.
+  if (S == 1) {
+    // single stage: HBM -> registers -> HBM (the middle pass applies the
+    // Green's function and the inverse DFT in registers)
+    WSIM_RADIX_SWITCH(fs[0].R,
       float2 v[P::template its<R>() * C * R];
-      if (s == 0) {
-        p.template stage_load_global<R>(v, st_);
-      } else {
-        p.template stage_load_lds<R>(v, st_, tw);
-        if (!last || mid) __syncthreads();
-      }
+      p.template stage_load_global<R>(v, fs[0]);
       p.template stage_compute<R, D0>(v);
-      if (!last) {
-        p.template stage_store_lds<R>(v, st_);
-        __syncthreads();
-      } else if (!mid) {
-        p.template stage_store_global<R>(v, st_);
-      } else {
-        // middle pass: Green's function in registers, then the inverse
-        // transform's first stage (same radix, Ns = 1, no twiddles)
-        p.template stage_green<R>(v, st_);
+      if (MID) {
+        p.template stage_green<R>(v, fs[0]);
         p.template stage_compute<R, 1>(v);
-        if (S == 1) {
-          p.template stage_store_global<R>(v, pl.i[0]);
-        } else {
-          p.template stage_store_lds<R>(v, pl.i[0]);
-          __syncthreads();
-        }
+        p.template stage_store_global<R>(v, pl.i[0]);
+      } else {
+        p.template stage_store_global<R>(v, fs[0]);
       }
     )
-  }
-  // ---- remaining inverse stages of the middle pass --------------------------
-  if (mid) {
-    for (int s = 1; s < S; ++s) {
-      const Stage st_ = pl.i[s];
-      const bool last = s == S - 1;
+  } else {
+    // first stage: HBM -> registers -> LDS
+    WSIM_RADIX_SWITCH(fs[0].R,
+      float2 v[P::template its<R>() * C * R];
+      p.template stage_load_global<R>(v, fs[0]);
+      p.template stage_compute<R, D0>(v);
+      p.template stage_store_lds<R>(v, fs[0]);
+    )
+    __syncthreads();
+    // middle stages: LDS -> registers -> LDS
+    for (int s = 1; s < S - 1; ++s) {
+      const Stage st_ = fs[s];
       WSIM_RADIX_SWITCH(st_.R,
         float2 v[P::template its<R>() * C * R];
-        p.template stage_load_lds<R>(v, st_, a.twi);
-        if (!last) __syncthreads();
+        p.template stage_load_lds<R>(v, st_, tw);
+        __syncthreads();
+        p.template stage_compute<R, D0>(v);
+        p.template stage_store_lds<R>(v, st_);
+      )
+      __syncthreads();
+    }
+    const Stage sl = fs[S - 1];
+    if (!MID) {
+      // last stage: LDS -> registers -> HBM
+      WSIM_RADIX_SWITCH(sl.R,
+        float2 v[P::template its<R>() * C * R];
+        p.template stage_load_lds<R>(v, sl, tw);
+        p.template stage_compute<R, D0>(v);
+        p.template stage_store_global<R>(v, sl);
+      )
+    } else {
+      // forward last stage, Green's function, inverse first stage (same
+      // radix, Ns = 1, no twiddles), back to LDS
+      WSIM_RADIX_SWITCH(sl.R,
+        float2 v[P::template its<R>() * C * R];
+        p.template stage_load_lds<R>(v, sl, tw);
+        __syncthreads();
+        p.template stage_compute<R, -1>(v);
+        p.template stage_green<R>(v, sl);
         p.template stage_compute<R, 1>(v);
-        if (!last) {
-          p.template stage_store_lds<R>(v, st_);
+        p.template stage_store_lds<R>(v, pl.i[0]);
+      )
+      __syncthreads();
+      for (int s = 1; s < S - 1; ++s) {
+        const Stage st_ = pl.i[s];
+        WSIM_RADIX_SWITCH(st_.R,
+          float2 v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R>(v, st_, twi_l);
           __syncthreads();
-        } else {
-          p.template stage_store_global<R>(v, st_);
-        }
+          p.template stage_compute<R, 1>(v);
+          p.template stage_store_lds<R>(v, st_);
+        )
+        __syncthreads();
+      }
+      const Stage il = pl.i[S - 1];
+      WSIM_RADIX_SWITCH(il.R,
+        float2 v[P::template its<R>() * C * R];
+        p.template stage_load_lds<R>(v, il, twi_l);
+        p.template stage_compute<R, 1>(v);
+        p.template stage_store_global<R>(v, il);
       )
     }
   }

@@ -21,6 +21,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
+from . import plans
 from .solver import MBSSolver
 
 ALPHA = 0.7  # f_mix.m:3
@@ -302,7 +303,18 @@ class WaveSimBase:
                                      complex(self.k02e), self.roi, self.energy_calculation_interval,
                                      nrhs=nrhs)
             self._solver_nrhs = nrhs
+            plan = plans.lookup(self.N, nrhs, len(self.wiggles))
+            if plan:
+                self._solver.plan = plan
         return self._solver
+
+    def tune(self, nrhs: int = 1, reps: int = 4):
+        """Autotune the FFT plan of this problem (radix order and tile width
+        per axis) and remember it for later solvers of the same shape."""
+        sv = self.solver(nrhs)
+        plan, ms = sv.tune(reps)
+        plans.store(self.N, nrhs, len(self.wiggles), plan, ms)
+        return plan, ms
 
     def _prepare(self, source: Source) -> Source:
         """Simulation.m:154-156: shift to the ROI origin and crop to the grid."""

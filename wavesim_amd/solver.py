@@ -125,6 +125,25 @@ class MBSSolver:
         capi.check(self._lib.wsim_solver_propagate(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
         return out
 
+    # -- FFT plans ---------------------------------------------------------------
+    def tune(self, reps: int = 4):
+        """Autotune the per-axis radix order and tile width on this problem;
+        returns (plan text, ms per iteration)."""
+        buf = C.create_string_buffer(512)
+        ms = C.c_double()
+        capi.check(self._lib.wsim_solver_tune(self._h, int(reps), buf, 512, C.byref(ms)))
+        return buf.value.decode(), ms.value
+
+    @property
+    def plan(self) -> str:
+        buf = C.create_string_buffer(512)
+        capi.check(self._lib.wsim_solver_get_plan(self._h, buf, 512))
+        return buf.value.decode()
+
+    @plan.setter
+    def plan(self, text: str):
+        capi.check(self._lib.wsim_solver_set_plan(self._h, text.encode()))
+
     def profile(self, reps: int = 10):
         """[(kernel name, mean ms, algorithmic bytes)] for one iteration."""
         K = 16
