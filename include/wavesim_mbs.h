@@ -138,6 +138,20 @@ int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, d
 int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
 int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);
 int wsim_solver_set_plan(wsim_solver* s, const char* plan);
+/* Warm start (no counterpart in the reference): the next solves start rhs
+ * from the field E0 (unprimed, N[0..3] entries, host or device memory)
+ * instead of 0.  The first iteration then adds the whole source i/eps*S and
+ * computes the increment  dE'_1 = a g^2 [G_1(V E0 + S) - E0]  of the
+ * recurrence of WaveSimBase.m:96-124, with E' = g E0 + dE'_1; later
+ * iterations are the reference loop.  Rhs without a warm start begin from
+ * E0 = 0 in the same mode.  wsim_solver_clear_warm_start returns to the
+ * reference start (source split over the first Nw iterations). */
+int wsim_solver_set_warm_start(wsim_solver* s, int rhs, const wsim_cfloat* E0, int on_device);
+int wsim_solver_clear_warm_start(wsim_solver* s);
+/* Stop-test denominator of one rhs (default <= 0: diff_energy(1) as in
+ * Simulation.m:198).  A fixed value makes the relative residual independent
+ * of the start, for comparing warm and zero starts at equal accuracy. */
+int wsim_solver_set_reference_energy(wsim_solver* s, int rhs, double energy);
 int wsim_solver_destroy(wsim_solver* s);
 
 const char* wsim_last_error(void);

@@ -564,11 +564,14 @@ class State:
 
 
 def next_state(sim: Sim, state: State, can_terminate: bool,
-               callback: Optional[Callable] = None) -> None:
-    """Simulation.next (Simulation.m:193-217)."""
+               callback: Optional[Callable] = None, ref_energy: Optional[float] = None) -> None:
+    """Simulation.next (Simulation.m:193-217).  ``ref_energy`` replaces the
+    denominator diff_energy(1) (no reference counterpart: a start-independent
+    relative residual for warm starts)."""
     state.diff_energy.append(state.last_step_energy)
     de = state.diff_energy
-    if can_terminate and de[-1] / de[0] < sim.energy_threshold:
+    den = ref_energy if ref_energy is not None and ref_energy > 0 else de[0]
+    if can_terminate and de[-1] / den < sim.energy_threshold:
         state.has_next = False
         state.converged = True
     elif can_terminate and state.it >= state.max_iterations:
@@ -583,10 +586,19 @@ def next_state(sim: Sim, state: State, can_terminate: bool,
 
 
 def run(sim: Sim, source: Source, workers=None, callback: Optional[Callable] = None,
-        max_iterations: Optional[int] = None, record_every: bool = False):
+        max_iterations: Optional[int] = None, record_every: bool = False,
+        E0: Optional[np.ndarray] = None, ref_energy: Optional[float] = None):
     """Simulation.exec (Simulation.m:138-190) + WaveSimBase.run_algorithm
     MATLAB branch (WaveSimBase.m:157-197).  Returns (E_roi, state); the
-    full-grid primed field E' is ``state.E_primed``."""
+    full-grid primed field E' is ``state.E_primed``.
+
+    Warm start (no reference counterpart): with ``E0`` (unprimed, full grid)
+    the loop starts from dE' = gamma E0, E' = 0; iteration 1 adds the whole
+    source (i/eps) S with wiggle 1 and computes the increment of the
+    recurrence of WaveSimBase.m:96-124,
+        dE'_1 = -a g (g E0) - i a g^2 G'(g E0 + i/eps S) = a g^2 [G(V E0 + S) - E0],
+    with E' = g E0 + dE'_1; later iterations add no source.  With Nw = 1 and
+    E0 = 0 this is exactly the zero-start loop."""
     src = prepare_source(sim, source)
     state = State()
     state.source_energy = src.energy()
@@ -606,20 +618,30 @@ def run(sim: Sim, source: Source, workers=None, callback: Optional[Callable] = N
     else:
         A = 1j / float(sim.epsilon) / nw
     g4 = sim.gamma[..., None]
+    warm = E0 is not None
+    if warm:
+        dE = (g4 * np.asarray(E0).reshape(N)).astype(cdt)
+        A_warm = (np.complex64(1j) / np.float32(sim.epsilon)) if sim.single else 1j / float(sim.epsilon)
     while state.has_next:
         wig = sim.wiggles[(state.it - 1) % nw]
-        if state.it <= nw:
-            Etmp = src.add_to(dE.copy(), A)
+        if warm and state.it == 1:
+            Etmp = propagate(sim, src.add_to(dE.copy(), A_warm), wig, workers)
+            inc = f_mix(dE, Etmp, g4, sim.single) - dE
+            E = E + dE + inc
+            dE = inc
         else:
-            Etmp = dE
-        Etmp = propagate(sim, Etmp, wig, workers)
-        dE = f_mix(dE, Etmp, g4, sim.single)
-        E = E + dE
+            if state.it <= nw and not warm:
+                Etmp = src.add_to(dE.copy(), A)
+            else:
+                Etmp = dE
+            Etmp = propagate(sim, Etmp, wig, workers)
+            dE = f_mix(dE, Etmp, g4, sim.single)
+            E = E + dE
         if record_every:
             state.energy_every.append(energy(crop(sim, dE)))
         if state.calculate_energy:
             state.last_step_energy = energy(crop(sim, dE))
-        next_state(sim, state, state.it % nw == 0, callback)
+        next_state(sim, state, state.it % nw == 0, callback, ref_energy)
     state.E_primed = E
     with np.errstate(divide="ignore", invalid="ignore"):
         Ef = E / g4

@@ -1,0 +1,99 @@
+"""Learned warm start kernels vs the CPU oracle (oracle/fno.py).
+
+The oracle rounds both operands of every matrix product to bfloat16 as the
+MFMA GEMM does, so forward outputs and gradients agree to FP32-accumulation
+level (rel-L2 <= 2e-3, a few operands may round across a BF16 boundary
+differently); against the unrounded float64 oracle the bar is 3e-2."""
+import numpy as np
+import pytest
+
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+
+
+@pytest.mark.parametrize("M,N,K,transA,transB,nb", [
+    (64, 64, 32, False, False, (1, 1, 1)),
+    (37, 91, 45, False, False, (1, 1, 1)),
+    (100, 33, 70, True, False, (2, 1, 1)),
+    (32, 130, 16, False, True, (2, 3, 1)),
+    (5, 7, 300, True, True, (1, 2, 3)),
+])
+def test_gemm_matches_numpy(gpu, M, N, K, transA, transB, nb):
+    from oracle.fno import round_bf16
+    from wavesim_amd.learned import kernels as Kn
+    rng = np.random.default_rng(0)
+    nbt = int(np.prod(nb))
+    A = rng.standard_normal((nbt, K, M) if transA else (nbt, M, K)).astype(np.float32)
+    B = rng.standard_normal((nbt, N, K) if transB else (nbt, K, N)).astype(np.float32)
+    Cm = rng.standard_normal((nbt, M, N)).astype(np.float32)
+    sA = (1, M) if transA else (K, 1)
+    sB = (1, K) if transB else (N, 1)
+    def bstr(per):
+        return (per * nb[1] * nb[2], per * nb[2], per)
+    dA, dB, dC = _t(A), _t(B), _t(Cm)
+    Kn.gemm(M, N, K, dA, sA + bstr(M * K), dB, sB + bstr(K * N), dC, (N, 1) + bstr(M * N),
+            alpha=0.5, beta=-1.0, nb=nb)
+    An = A.transpose(0, 2, 1) if transA else A
+    Bn = B.transpose(0, 2, 1) if transB else B
+    ref = 0.5 * np.einsum("bmk,bkn->bmn", round_bf16(An), round_bf16(Bn)) - Cm
+    out = dC.cpu().numpy()
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-4 * np.abs(ref).max())
+
+
+def _pair(seed=0):
+    from oracle import fno as O
+    from wavesim_amd.learned import fno as G
+    cfgo = O.Config(L=16, cin=4, width=8, modes=4, layers=2, proj=16)
+    cfgg = G.Config(L=16, cin=4, width=8, modes=4, layers=2, proj=16)
+    p = O.init_params(cfgo, seed)
+    rng = np.random.default_rng(seed + 1)
+    for k in p:
+        p[k] = p[k] + 0.2 * rng.standard_normal(p[k].shape)
+    x = rng.standard_normal((2, 4, 16, 16, 16))
+    t = rng.standard_normal((2, 2, 16, 16, 16))
+    return O, cfgo, G.FNO(cfgg, p), p, x, t
+
+
+def test_fno_forward_matches_oracle(gpu):
+    O, cfgo, net, p, x, t = _pair()
+    y, _ = net.forward(_t(x.reshape(2, 4, -1)))
+    yo, _ = O.Net(cfgo, bf16=True).forward(p, x)
+    y64, _ = O.Net(cfgo, bf16=False).forward(p, x)
+    y = y.cpu().numpy().reshape(yo.shape)
+    assert rel_l2(y, yo) <= 2e-3
+    assert rel_l2(y, y64) <= 3e-2
+
+
+def test_fno_gradients_match_oracle(gpu):
+    O, cfgo, net, p, x, t = _pair(3)
+    loss, _ = net.loss_and_grad(_t(x.reshape(2, 4, -1)), _t(t.reshape(2, 2, -1)))
+    Lo, go, _ = O.Net(cfgo, bf16=True).loss(p, x, t)
+    _, g64, _ = O.Net(cfgo, bf16=False).loss(p, x, t)
+    assert abs(loss - Lo) <= 1e-3 * abs(Lo)
+    gg = net.grads()
+    for k in go:
+        e = rel_l2(gg[k], go[k])
+        e64 = rel_l2(gg[k], g64[k])
+        print(k, f"{e:.2e} {e64:.2e}", flush=True)
+        assert e <= 2e-3, k
+        assert e64 <= 3e-2, k
+
+
+def test_adam_step_matches_formula(gpu):
+    import torch
+    O, cfgo, net, p, x, t = _pair(5)
+    net.loss_and_grad(_t(x.reshape(2, 4, -1)), _t(t.reshape(2, 2, -1)))
+    p0 = net.flat.cpu().numpy().copy()
+    g = net.grad.cpu().numpy().copy()
+    net.adam_step(lr=1e-2)
+    p1 = net.flat.cpu().numpy()
+    m = 0.1 * g
+    v = 0.001 * g * g
+    ref = p0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    assert np.allclose(p1, ref, rtol=1e-5, atol=1e-6)

@@ -221,3 +221,29 @@ def test_plan_rejects_bad_text(gpu):
     for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y"):
         with pytest.raises(WsimError):
             sv.plan = bad
+
+
+@pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_periodic_2d, cases.case_3d_disordered])
+def test_warm_start_matches_oracle(gpu, builder):
+    """Warm start from a smooth E0: same iterations and field as the oracle's
+    restatement of the warm-start increment recurrence."""
+    from oracle import mbs
+    import wavesim_amd as w
+    n, opt, src = builder()
+    o, g = _pair(n, opt)
+    rng = np.random.default_rng(11)
+    E0 = (rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)) * 1e-3
+    Ez, sz = mbs.run(o, cases.oracle_source(src))
+    Eo, so = mbs.run(o, cases.oracle_source(src), E0=E0, ref_energy=sz.diff_energy[0], record_every=True)
+    sv = g.solver()
+    sv.set_warm_start(0, E0)
+    sv.set_reference_energy(0, sz.diff_energy[0])
+    Eg, sg = g.exec(cases.gpu_source(src))
+    sv.clear_warm_start()
+    sv.set_reference_energy(0, 0.0)
+    assert sg.iterations == so.it - 1
+    assert rel_l2(Eg, Eo) <= TOL
+    assert np.allclose(sg.energy_every, so.energy_every, rtol=1e-4, atol=1e-4 * max(so.energy_every))
+    # back to the reference start
+    Eg2, sg2 = g.exec(cases.gpu_source(src))
+    assert sg2.iterations == sz.it - 1 and rel_l2(Eg2, Ez) <= TOL

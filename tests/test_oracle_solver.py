@@ -179,3 +179,48 @@ def test_source_outside_grid_is_cropped():
     sim = mbs.make_sim(np.ones((8, 8)), dict(pixel_size=0.25, boundary_widths=[2, 2]))
     s = mbs.prepare_source(sim, mbs.Source.point(np.ones((4, 1)), [-5, 3]))
     assert s.positions == []
+
+
+def test_warm_start_zero_equals_zero_start_nw1():
+    n, opt, src = cases.case_periodic_2d(thr=1e-8)
+    sim = mbs.make_sim(n, dict(opt, single_precision=False))
+    assert len(sim.wiggles) == 1
+    E1, s1 = mbs.run(sim, cases.oracle_source(src))
+    E2, s2 = mbs.run(sim, cases.oracle_source(src), E0=np.zeros(sim.N))
+    assert s1.it == s2.it and rel_l2(E2, E1) < 1e-13
+
+
+def test_warm_start_from_solution_stops_at_once_nw1():
+    n, opt, src = cases.case_periodic_2d(thr=1e-14)
+    sim = mbs.make_sim(n, dict(opt, single_precision=False))
+    E, s = mbs.run(sim, cases.oracle_source(src))
+    sim2 = mbs.make_sim(n, dict(opt, single_precision=False, energy_threshold=1e-6))
+    Ew, sw = mbs.run(sim2, cases.oracle_source(src), E0=s.E_primed / sim.gamma[..., None],
+                     ref_energy=s.diff_energy[0])
+    assert sw.it - 1 == 1                       # the first stop test passes
+    assert rel_l2(Ew, E) < 1e-6
+
+
+def test_warm_start_with_wiggles_close_to_zero_start():
+    """With Nw > 1 the zero-start answer is not the fixed point of one wiggle's
+    step, so a warm start converges to it only up to the wrap-around
+    correction (here 8-px boundaries: < 2%)."""
+    n, opt, src = cases.case_2d_disordered(thr=1e-12)
+    sim = mbs.make_sim(n, dict(opt, single_precision=False))
+    E, s = mbs.run(sim, cases.oracle_source(src))
+    with np.errstate(all="ignore"):
+        Efull = s.E_primed / sim.gamma[..., None]
+    Efull[~np.isfinite(Efull)] = 0
+    sim2 = mbs.make_sim(n, dict(opt, single_precision=False, energy_threshold=1e-8))
+    Ew, sw = mbs.run(sim2, cases.oracle_source(src), E0=Efull, ref_energy=s.diff_energy[0])
+    assert sw.converged and rel_l2(Ew, E) < 2e-2
+
+
+def test_warm_start_converges_to_fixed_point_nw1():
+    n, opt, src = cases.case_periodic_2d(thr=1e-14)
+    sim = mbs.make_sim(n, dict(opt, single_precision=False))
+    E, s = mbs.run(sim, cases.oracle_source(src))
+    rng = np.random.default_rng(0)
+    E0 = 1e-3 * (rng.standard_normal(sim.N) + 1j * rng.standard_normal(sim.N))
+    Ew, sw = mbs.run(sim, cases.oracle_source(src), E0=E0, ref_energy=s.diff_energy[0])
+    assert rel_l2(Ew, E) < 1e-6

@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwavesim_mbs.so")
+FNO_LIB = os.path.join(HERE, "libwavesim_fno.so")
 ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
 
 # (contig, mode, prologue, epilogue) -- must match WSIM_FOR_EACH_VARIANT
@@ -58,8 +59,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
             units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
                           [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",
                            f"-DWSIM_EPI={e}", f"-DWSIM_C={c}"]))
+    fno_obj = os.path.join(OBJ, "fno.o")
     jobs = jobs or min(16, os.cpu_count() or 4)
     cmds = [[hipcc()] + _flags() + defs + ["-c", src, "-o", obj] for src, obj, defs in units]
+    cmds.append([hipcc()] + _flags() + ["-c", os.path.join(CSRC, "fno.hip"), "-o", fno_obj])
     with cf.ThreadPoolExecutor(jobs) as ex:
         for cmd, msg in zip(cmds, ex.map(_run, cmds)):
             if verbose:
@@ -69,6 +72,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [u[1] for u in units])
     os.replace(tmp, LIB)
+    tmp = FNO_LIB + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, fno_obj])
+    os.replace(tmp, FNO_LIB)
     return LIB
 
 

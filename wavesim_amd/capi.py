@@ -67,6 +67,9 @@ SYMBOLS = {
     "wsim_solver_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "wsim_solver_get_plan": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "wsim_solver_set_plan": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "wsim_solver_set_warm_start": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "wsim_solver_clear_warm_start": (C.c_int, [C.c_void_p]),
+    "wsim_solver_set_reference_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "wsim_solver_destroy": (C.c_int, [C.c_void_p]),
     "wsim_last_error": (C.c_char_p, []),
     "wsim_abi_version": (C.c_int, []),

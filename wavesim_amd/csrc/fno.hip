@@ -1,0 +1,327 @@
+// Kernels of the learned warm start (factorised 3-D Fourier neural operator).
+//
+// Everything the network needs, forward and backward, is one of:
+//   * wfno_gemm: strided, two-level-batched real GEMM  C = alpha A B + beta C
+//     on BF16 matrix cores (v_mfma_f32_32x32x8_bf16, FP32 accumulate); A and B
+//     are FP32 in memory with arbitrary strides and are rounded to BF16
+//     (round-to-nearest-even) when staged into LDS.  Transposes are stride
+//     swaps, so the backward GEMMs (dA = dC B^T, dB = A^T dC) use the same kernel.
+//     The truncated DFTs of the spectral layers are GEMMs with cos/sin matrices.
+//   * elementwise kernels: bias + GELU forward, GELU backward, per-channel bias
+//     gradient, MSE loss gradient, Adam update, and the composition of the warm
+//     start field from the predicted envelope.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+#include "wavesim_fno.h"
+
+namespace wfno {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+static thread_local std::string g_err;
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;  // LDS row pitch in bf16 (80 B)
+
+// Workgroup: 4 waves, each a 32x32 output block of the 64x64 tile.
+__global__ void __launch_bounds__(256) gemm_kernel(wfno_gemm_args g) {
+  __shared__ unsigned short As[TM * KP];  // [m][k]
+  __shared__ unsigned short Bs[TN * KP];  // [n][k]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long long m0 = (long long)blockIdx.y * TM, n0 = (long long)blockIdx.x * TN;
+  const long long b3 = blockIdx.z % g.nb3, b12 = blockIdx.z / g.nb3;
+  const long long b1 = b12 / g.nb2, b2 = b12 % g.nb2;
+  const float* A = g.A + b1 * g.sab1 + b2 * g.sab2 + b3 * g.sab3;
+  const float* B = g.B + b1 * g.sbb1 + b2 * g.sbb2 + b3 * g.sbb3;
+  float* Cm = g.C + b1 * g.scb1 + b2 * g.scb2 + b3 * g.scb3;
+
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+  for (long long k0 = 0; k0 < g.K; k0 += TK) {
+    // stage A[m0:m0+64, k0:k0+32] and B[k0:k0+32, n0:n0+64] as bf16
+    for (int e = tid; e < TM * TK; e += 256) {
+      int mi, ki;
+      if (g.sak == 1) { mi = e / TK; ki = e % TK; } else { ki = e / TM; mi = e % TM; }
+      const long long m = m0 + mi, k = k0 + ki;
+      const float v = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.f;
+      As[mi * KP + ki] = to_bf16(v);
+    }
+    for (int e = tid; e < TN * TK; e += 256) {
+      int ni, ki;
+      if (g.sbk == 1) { ni = e / TK; ki = e % TK; } else { ki = e / TN; ni = e % TN; }
+      const long long n = n0 + ni, k = k0 + ki;
+      const float v = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.f;
+      Bs[ni * KP + ki] = to_bf16(v);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 8) {
+      // lane holds A[i][kk + 4*(lane/32) + j] and B[kk + 4*(lane/32) + j][n], j = 0..3
+      const int i = wm * 32 + (lane & 31);
+      const int n = wn * 32 + (lane & 31);
+      const int kb = kk + 4 * (lane >> 5);
+      bf16x4 a, b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = (short)As[i * KP + kb + j];
+        b[j] = (short)Bs[n * KP + kb + j];
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // acc[r] holds row 8*(r/4) + 4*(lane/32) + r%4, column lane%32 of the wave's block
+  const long long n = n0 + wn * 32 + (lane & 31);
+  if (n >= g.N) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long long m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m < g.M) {
+      float* c = Cm + m * g.scm + n * g.scn;
+      *c = g.alpha * acc[r] + (g.beta != 0.f ? g.beta * *c : 0.f);
+    }
+  }
+}
+
+__global__ void reduce_chunks_kernel(const float* src, float* dst, int nchunks, long long n, int accumulate) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s = accumulate ? dst[i] : 0.f;
+    for (int c = 0; c < nchunks; ++c) s += src[(long long)c * n + i];
+    dst[i] = s;
+  }
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(s (x + 0.044715 x^3))), s = sqrt(2/pi)
+__device__ __forceinline__ float gelu(float x) {
+  const float s = 0.7978845608028654f;
+  return 0.5f * x * (1.f + tanhf(s * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float s = 0.7978845608028654f;
+  const float u = s * (x + 0.044715f * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * s * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// pre[c][v] += bias[c]; out[c][v] = act ? gelu(pre) : pre   (batch: stride C*V)
+__global__ void bias_act_kernel(float* pre, float* out, const float* bias, long long C, long long V,
+                                long long total, int act) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long c = (i / V) % C;
+    const float x = pre[i] + bias[c];
+    pre[i] = x;
+    out[i] = act ? gelu(x) : x;
+  }
+}
+
+// dpre = dout * gelu'(pre)
+__global__ void gelu_bwd_kernel(const float* dout, const float* pre, float* dpre, long long total) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x)
+    dpre[i] = dout[i] * gelu_grad(pre[i]);
+}
+
+// dbias[c] (+)= sum over batch and v of d[b][c][v]; one block per channel, fixed order
+__global__ void __launch_bounds__(256) bias_grad_kernel(const float* d, float* dbias, long long C, long long V,
+                                                        long long nb, int accumulate) {
+  __shared__ double red[256];
+  const long long c = blockIdx.x;
+  double s = 0.0;
+  for (long long b = 0; b < nb; ++b)
+    for (long long v = threadIdx.x; v < V; v += 256) s += d[(b * C + c) * V + v];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dbias[c] = (accumulate ? dbias[c] : 0.f) + (float)red[0];
+}
+
+// loss partials of 0.5 mean((pred - target)^2) and its gradient
+__global__ void __launch_bounds__(256) mse_kernel(const float* pred, const float* target, float* dpred,
+                                                  double* partial, long long total, float scale) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const float e = pred[i] - target[i];
+    dpred[i] = e * scale;
+    s += (double)e * e;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction
+__global__ void adam_kernel(float* p, const float* g, float* m, float* v, long long n, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// E0[y,x,z] = u(z) * (1 + A(y/f, x/f, z/f)) inside the ROI box, u(z) on the
+// source footprint outside it, 0 elsewhere.  A is [2][Z][Y][X] at 1/f resolution
+// (channel 0 real, 1 imaginary; x fastest), u is the prior along z.
+__global__ void compose_warm_kernel(float2* E0, const float* A, const float2* u, int Ny, int Nx, int Nz,
+                                    int y0, int x0, int z0, int ny, int nx, int nz, int f, int fy0, int fy1,
+                                    int fx0, int fx1) {
+  const long long total = (long long)Ny * Nx * Nz;
+  const int Xl = nx / f, Yl = ny / f, Zl = nz / f;
+  const long long plane = (long long)Xl * Yl * Zl;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(i % Ny);
+    const int x = (int)((i / Ny) % Nx);
+    const int z = (int)(i / ((long long)Ny * Nx));
+    float2 v = make_float2(0.f, 0.f);
+    if (y >= fy0 && y < fy1 && x >= fx0 && x < fx1) v = u[z];
+    const int ly = y - y0, lx = x - x0, lz = z - z0;
+    if (ly >= 0 && ly < ny && lx >= 0 && lx < nx && lz >= 0 && lz < nz) {
+      const long long j = ((long long)(lz / f) * Yl + ly / f) * Xl + lx / f;
+      const float ar = 1.f + A[j], ai = A[plane + j];
+      v = make_float2(v.x * ar - v.y * ai, v.x * ai + v.y * ar);
+    }
+    E0[i] = v;
+  }
+}
+
+static unsigned grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+template <class F>
+static int guard(F&& f) {
+  const int rc = f();
+  if (rc != 0) return rc;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_err = hipGetErrorString(e);
+    return -2;
+  }
+  return 0;
+}
+
+}  // namespace wfno
+
+using namespace wfno;
+
+extern "C" {
+
+const char* wfno_last_error(void) { return g_err.c_str(); }
+
+int wfno_gemm(const wfno_gemm_args* a, void* stream) {
+  return guard([&]() -> int {
+    if (!a || a->M <= 0 || a->N <= 0 || a->K <= 0 || a->nb1 <= 0 || a->nb2 <= 0 || a->nb3 <= 0) {
+      g_err = "bad gemm shape";
+      return -1;
+    }
+    const long long nb = (long long)a->nb1 * a->nb2 * a->nb3;
+    if (nb > 65535 || (a->M + TM - 1) / TM > 65535) {
+      g_err = "gemm grid too large";
+      return -1;
+    }
+    dim3 grid((unsigned)((a->N + TN - 1) / TN), (unsigned)((a->M + TM - 1) / TM), (unsigned)nb);
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, (hipStream_t)stream, *a);
+    return 0;
+  });
+}
+
+int wfno_reduce_chunks(const float* src, float* dst, int nchunks, long long n, int accumulate, void* stream) {
+  return guard([&]() -> int {
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, src, dst,
+                       nchunks, n, accumulate);
+    return 0;
+  });
+}
+
+int wfno_bias_act(float* pre, float* out, const float* bias, long long C, long long V, long long nb, int act,
+                  void* stream) {
+  return guard([&]() -> int {
+    const long long n = C * V * nb;
+    hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, pre, out, bias,
+                       C, V, n, act);
+    return 0;
+  });
+}
+
+int wfno_gelu_bwd(const float* dout, const float* pre, float* dpre, long long n, void* stream) {
+  return guard([&]() -> int {
+    hipLaunchKernelGGL(gelu_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, dout, pre, dpre, n);
+    return 0;
+  });
+}
+
+int wfno_bias_grad(const float* d, float* dbias, long long C, long long V, long long nb, int accumulate,
+                   void* stream) {
+  return guard([&]() -> int {
+    hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)C), dim3(256), 0, (hipStream_t)stream, d, dbias, C, V,
+                       nb, accumulate);
+    return 0;
+  });
+}
+
+int wfno_mse(const float* pred, const float* target, float* dpred, double* partial, int nblocks, long long n,
+             float scale, void* stream) {
+  return guard([&]() -> int {
+    hipLaunchKernelGGL(mse_kernel, dim3((unsigned)nblocks), dim3(256), 0, (hipStream_t)stream, pred, target,
+                       dpred, partial, n, scale);
+    return 0;
+  });
+}
+
+int wfno_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+              int t, void* stream) {
+  return guard([&]() -> int {
+    const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, p, g, m, v, n, lr, b1,
+                       b2, eps, c1, c2);
+    return 0;
+  });
+}
+
+int wfno_compose_warm(void* E0, const float* A, const void* u, const int grid[3], const int roi_lo[3],
+                      const int roi_n[3], int factor, const int foot[4], void* stream) {
+  return guard([&]() -> int {
+    if (factor < 1 || roi_n[0] % factor || roi_n[1] % factor || roi_n[2] % factor) {
+      g_err = "ROI not divisible by the network's down-sampling factor";
+      return -1;
+    }
+    const long long n = (long long)grid[0] * grid[1] * grid[2];
+    hipLaunchKernelGGL(compose_warm_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)E0, A,
+                       (const float2*)u, grid[0], grid[1], grid[2], roi_lo[0], roi_lo[1], roi_lo[2], roi_n[0],
+                       roi_n[1], roi_n[2], factor, foot[0], foot[1], foot[2], foot[3]);
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -175,7 +175,8 @@ __global__ void __launch_bounds__(256) control_kernel(DevState* st, const double
       const double de = st->last_energy[r];
       st->diff[h] = de;
       const bool can = (it % st->nw) == 0;
-      const double d1 = st->diff[(size_t)r * st->hist_cap];
+      const double ref = st->ref_energy[r];
+      const double d1 = ref > 0.0 ? ref : st->diff[(size_t)r * st->hist_cap];
       if (can && de / d1 < st->threshold) {
         st->stopped[r] = 1;
         st->converged[r] = 1;
@@ -194,6 +195,13 @@ __global__ void __launch_bounds__(256) control_kernel(DevState* st, const double
     st->it = it + 1;
     st->all_stopped = all;
   }
+}
+
+// x[c][v] *= gamma[v] for the C components of one field (warm-start seeding)
+__global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2* g, long long nvox, int C) {
+  const long long n = nvox * C;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    x[i] = cmul(x[i], g[i % nvox]);
 }
 
 // ----------------------------------------------------------------------------
@@ -257,6 +265,10 @@ struct Solver {
   DevState* dprobe = nullptr;
   int *d_stopped = nullptr, *d_conv = nullptr, *d_niter = nullptr;
   double *d_last = nullptr, *d_diff = nullptr, *d_eall = nullptr;
+  double* d_ref = nullptr;              // [nrhs] stop-test denominators
+  float2* W0 = nullptr;                 // [nrhs][C][nvox] gamma * E0 of warm starts
+  std::vector<char> has_warm;           // per rhs
+  bool warm_mode = false;
   int *p_stopped = nullptr;  // probe state arrays
   double* p_dummy = nullptr;
   int hist_cap = 0;
@@ -285,7 +297,7 @@ struct Solver {
     gexec = nullptr;
     graph = nullptr;
     void* ptrs[] = {gamma, E, dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
-                    d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args};
+                    d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args, d_ref, W0};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (int a = 0; a < 3; ++a) {
@@ -753,6 +765,12 @@ struct Solver {
     dalloc(&d_conv, nrhs);
     dalloc(&d_niter, nrhs);
     dalloc(&d_last, nrhs);
+    dalloc(&d_ref, nrhs);
+    {
+      std::vector<double> z(nrhs, 0.0);
+      HIPCHK(hipMemcpy(d_ref, z.data(), nrhs * sizeof(double), hipMemcpyHostToDevice));
+    }
+    has_warm.assign(nrhs, 0);
     dalloc(&p_stopped, 1);
     HIPCHK(hipMemset(p_stopped, 0, sizeof(int)));
     dalloc(&p_dummy, 1);
@@ -805,6 +823,31 @@ struct Solver {
     has_source[rhs] = any ? 1 : 0;
   }
 
+  // warm start of one rhs: W0 = gamma * E0 (E0 unprimed, [pol][z][x][y])
+  void set_warm(int rhs, const wsim_cfloat* E0, bool on_device) {
+    require(rhs >= 0 && rhs < nrhs && E0 != nullptr, "bad argument");
+    const size_t f1 = (size_t)field_elems();
+    if (!W0) dalloc(&W0, f1 * nrhs);
+    float2* dst = W0 + rhs * f1;
+    HIPCHK(hipMemcpy(dst, E0, f1 * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    const unsigned blocks = (unsigned)std::min<size_t>((f1 + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(gamma_mul_kernel, dim3(blocks), dim3(256), 0, stream, dst, gamma, (long long)nvox, C);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    has_warm[rhs] = 1;
+    warm_mode = true;
+  }
+
+  void clear_warm() {
+    warm_mode = false;
+    std::fill(has_warm.begin(), has_warm.end(), 0);
+  }
+
+  void set_reference_energy(int rhs, double e) {
+    require(rhs >= 0 && rhs < nrhs, "rhs out of range");
+    HIPCHK(hipMemcpy(d_ref + rhs, &e, sizeof(double), hipMemcpyHostToDevice));
+  }
+
   void ensure_history(int cap) {
     if (cap <= hist_cap) return;
     if (d_diff) (void)hipFree(d_diff);
@@ -834,6 +877,10 @@ struct Solver {
     h.last_energy = d_last;
     h.diff = d_diff;
     h.energy_all = d_eall;
+    h.ref_energy = d_ref;
+    h.warm = warm_mode ? 1 : 0;
+    h.src_iters = warm_mode ? 1 : nw;
+    h.src_scale = warm_mode ? (float)nw : 1.f;
     std::vector<int> st(nrhs), zero(nrhs, 0);
     int all = 1;
     for (int r = 0; r < nrhs; ++r) {
@@ -849,7 +896,18 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(d_last, inf.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, stream));
     const size_t fe = (size_t)field_elems() * nrhs;
     HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
-    HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
+    if (warm_mode) {
+      // dE' starts at gamma E0 for warm-started rhs, 0 for the others
+      const size_t f1 = (size_t)field_elems();
+      for (int r = 0; r < nrhs; ++r) {
+        if (has_warm[r])
+          HIPCHK(hipMemcpyAsync(dE + r * f1, W0 + r * f1, f1 * sizeof(float2), hipMemcpyDeviceToDevice, stream));
+        else
+          HIPCHK(hipMemsetAsync(dE + r * f1, 0, f1 * sizeof(float2), stream));
+      }
+    } else {
+      HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
+    }
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -1165,6 +1223,30 @@ int wsim_solver_set_plan(wsim_solver* s, const char* plan) {
   return guard([&]() -> int {
     require(s != nullptr && plan != nullptr, "bad argument");
     s->s.set_plan(plan);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_warm_start(wsim_solver* s, int rhs, const wsim_cfloat* E0, int on_device) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.set_warm(rhs, E0, on_device != 0);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_clear_warm_start(wsim_solver* s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.clear_warm();
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_reference_energy(wsim_solver* s, int rhs, double energy) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.set_reference_energy(rhs, energy);
     return WSIM_OK;
   });
 }

@@ -74,12 +74,16 @@ struct DevState {
   int nblk;         // energy partials per rhs
   long long max_iter;
   double threshold;
+  int src_iters;      // iterations that add the source: Nw (zero start) or 1 (warm start)
+  float src_scale;    // multiplier of the stored i/eps/Nw * S: 1, or Nw for a warm start
+  int warm;           // iteration 1 is the first step from a warm start E0 (dE' = gamma E0)
   int* stopped;        // [nrhs]
   int* converged;      // [nrhs]
   int* n_iter;         // [nrhs]
   double* last_energy; // [nrhs]
   double* diff;        // [nrhs][hist_cap]  diff_energy as the reference records it
   double* energy_all;  // [nrhs][hist_cap]  |crop dE'|^2 of every iteration
+  double* ref_energy;  // [nrhs] stop-test denominator; <= 0: diff_energy(1) (Simulation.m:198)
 };
 
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
@@ -152,6 +156,8 @@ struct Pass {
   const ColInfo* cols;
   int tid;
   bool add_src;
+  float src_scale;
+  bool warm_step;  // first step from a warm start
   uint32_t lg_ncol;
   const float2* ramp_a;  // this wiggle's ramp along the axis
   const float* pe_a;
@@ -190,7 +196,7 @@ struct Pass {
     const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
     float2 v = ldg(in_b[c], i);
     if (PRO == PRO_SRC_RAMP) {
-      if (add_src) v = cadd(v, ldg(S_b[c], i));
+      if (add_src) v = cadd(v, cscale(ldg(S_b[c], i), src_scale));
       v = cmul(v, cmul(ramp_a[row], ci.rc));
     }
     return v;
@@ -322,9 +328,12 @@ struct Pass {
                 const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
                 const float2 d0 = ldg(dE_b[c], i);
                 const float2 e0 = ldg(E_b[c], i);
-                const float2 dn = csub(cmul(c1, d0), cmul(c2, ep));
+                // warm start: dE'_1 = -a g (g E0) - 0.7 i g^2 Eprop and
+                // E' = g E0 + dE'_1, with dE' holding g E0 on entry
+                const float2 dn = warm_step ? csub(cmul(make_float2(c1.x - 1.f, c1.y), d0), cmul(c2, ep))
+                                            : csub(cmul(c1, d0), cmul(c2, ep));
                 stg(dE_b[c], i, dn);
-                stg(E_b[c], i, cadd(e0, dn));
+                stg(E_b[c], i, warm_step ? cadd(cadd(e0, d0), dn) : cadd(e0, dn));
                 if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
                   eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
               }
@@ -490,7 +499,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   __syncthreads();
 
   using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT>;
-  P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw, 0u, ramp_l, pe_l};
+  P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= st->src_iters, st->src_scale,
+      st->warm != 0 && it_cur == 1, 0u, ramp_l, pe_l};
   p.eacc = 0.0;
   if (!CONTIG) {
     p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);

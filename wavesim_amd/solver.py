@@ -125,6 +125,22 @@ class MBSSolver:
         capi.check(self._lib.wsim_solver_propagate(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
         return out
 
+    # -- warm start -------------------------------------------------------------
+    def set_warm_start(self, rhs: int, E0=None, device_ptr: Optional[int] = None):
+        """Start rhs from the unprimed field E0 ([y,x,z,pol] host array) or
+        from a device buffer holding it in the solver's layout."""
+        if device_ptr is not None:
+            capi.check(self._lib.wsim_solver_set_warm_start(self._h, int(rhs), C.c_void_p(device_ptr), 1))
+        else:
+            x = capi.c64(np.asarray(E0).reshape(self.N))
+            capi.check(self._lib.wsim_solver_set_warm_start(self._h, int(rhs), capi.ptr(x), 0))
+
+    def clear_warm_start(self):
+        capi.check(self._lib.wsim_solver_clear_warm_start(self._h))
+
+    def set_reference_energy(self, rhs: int, energy: float):
+        capi.check(self._lib.wsim_solver_set_reference_energy(self._h, int(rhs), float(energy)))
+
     # -- FFT plans ---------------------------------------------------------------
     def tune(self, reps: int = 4):
         """Autotune the per-axis radix order and tile width on this problem;

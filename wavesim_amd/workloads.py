@@ -57,9 +57,28 @@ def sphere_vector_3d(n=128, boundary=32, radius=20, residual=1e-3):
     return nmap, opt, Source(np.ones((n, n, 1, 1)), [1, 1, 1, 1])
 
 
+def spheres_3d(n=128, boundary=32, seed=0, residual=1e-3, n_bg=1.33):
+    """1-8 seeded random spheres (radius 4-16 px, n 1.2-1.6) in a background
+    n_bg, plane-wave source: the plane z = 1 of the ROI (propagation along z)."""
+    rng = np.random.default_rng(seed)
+    nmap = np.full((n, n, n), n_bg, dtype=np.float32)
+    g = np.arange(n, dtype=np.float32)
+    for _ in range(int(rng.integers(1, 9))):
+        r = rng.uniform(4, 16)
+        c = rng.uniform(r, n - r, size=3)
+        ns = rng.uniform(1.2, 1.6)
+        d2 = ((g[:, None, None] - c[0]) ** 2 + (g[None, :, None] - c[1]) ** 2 +
+              (g[None, None, :] - c[2]) ** 2)
+        nmap[d2 <= r * r] = ns
+    opt = dict(pixel_size=0.25, boundary_widths=[boundary] * 3, energy_threshold=residual ** 2,
+               callback_interval=1e9)
+    return nmap, opt, Source(np.ones((n, n, 1)), [1, 1, 1])
+
+
 WORKLOADS = {
     "disordered_3d_256": lambda: disordered_3d(256, 32),
     "disordered_2d_1024": lambda: disordered_2d(1024, 64),
     "homogeneous_2d_256": lambda: homogeneous_2d(256, 32),
     "sphere_vector_3d_128": lambda: sphere_vector_3d(128, 32),
+    "spheres_3d_128": lambda: spheres_3d(128, 32),
 }
