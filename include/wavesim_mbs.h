@@ -138,14 +138,22 @@ int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, d
 int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
 int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);
 int wsim_solver_set_plan(wsim_solver* s, const char* plan);
+/* A new medium on the same grid and wiggle count: gamma, epsilon, k02e and
+ * the wiggle tables are replaced (as a new WaveSim would compute them,
+ * WaveSimBase.m:40-82); every rhs loses its source, set them again.  Lets one
+ * solver (and its device buffers) serve a sequence of media. */
+int wsim_solver_set_medium(wsim_solver* s, const wsim_cfloat* gamma, float epsilon, wsim_cfloat k02e,
+                           const wsim_wiggle* wiggles);
 /* Warm start (no counterpart in the reference): the next solves start rhs
  * from the field E0 (unprimed, N[0..3] entries, host or device memory)
- * instead of 0.  The first iteration then adds the whole source i/eps*S and
- * computes the increment  dE'_1 = a g^2 [G_1(V E0 + S) - E0]  of the
- * recurrence of WaveSimBase.m:96-124, with E' = g E0 + dE'_1; later
- * iterations are the reference loop.  Rhs without a warm start begin from
- * E0 = 0 in the same mode.  wsim_solver_clear_warm_start returns to the
- * reference start (source split over the first Nw iterations). */
+ * instead of 0.  With W0 = gamma E0: E' starts at W0 and during the first Nw
+ * iterations, alongside the source portions i/eps/Nw S of the reference loop
+ * (WaveSimBase.m:171-172), the correction W0/Nw enters the increment
+ * recurrence of WaveSimBase.m:96-124 through each wiggle in turn:
+ *     dE'_k = M_w (dE'_{k-1} + W0/Nw) - W0/Nw + B_w S/Nw,
+ * so E0 = 0 is exactly the reference loop and a good E0 leaves only small
+ * increments.  Rhs without a warm start have W0 = 0.
+ * wsim_solver_clear_warm_start returns to the reference start. */
 int wsim_solver_set_warm_start(wsim_solver* s, int rhs, const wsim_cfloat* E0, int on_device);
 int wsim_solver_clear_warm_start(wsim_solver* s);
 /* Stop-test denominator of one rhs (default <= 0: diff_energy(1) as in

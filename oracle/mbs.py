@@ -593,12 +593,11 @@ def run(sim: Sim, source: Source, workers=None, callback: Optional[Callable] = N
     full-grid primed field E' is ``state.E_primed``.
 
     Warm start (no reference counterpart): with ``E0`` (unprimed, full grid)
-    the loop starts from dE' = gamma E0, E' = 0; iteration 1 adds the whole
-    source (i/eps) S with wiggle 1 and computes the increment of the
-    recurrence of WaveSimBase.m:96-124,
-        dE'_1 = -a g (g E0) - i a g^2 G'(g E0 + i/eps S) = a g^2 [G(V E0 + S) - E0],
-    with E' = g E0 + dE'_1; later iterations add no source.  With Nw = 1 and
-    E0 = 0 this is exactly the zero-start loop."""
+    and W0 = gamma E0, E' starts at W0 and during the first Nw iterations the
+    correction W0/Nw enters the increment recurrence of WaveSimBase.m:96-124
+    through each wiggle in turn, next to the reference's source portions:
+        X = dE' + W0/Nw,  dE' <- f_mix(X, propagate(X + i/eps/Nw S)) - W0/Nw.
+    With E0 = 0 this is exactly the zero-start loop."""
     src = prepare_source(sim, source)
     state = State()
     state.source_energy = src.energy()
@@ -620,17 +619,18 @@ def run(sim: Sim, source: Source, workers=None, callback: Optional[Callable] = N
     g4 = sim.gamma[..., None]
     warm = E0 is not None
     if warm:
-        dE = (g4 * np.asarray(E0).reshape(N)).astype(cdt)
-        A_warm = (np.complex64(1j) / np.float32(sim.epsilon)) if sim.single else 1j / float(sim.epsilon)
+        W = (g4 * np.asarray(E0).reshape(N)).astype(cdt)
+        E = W.copy()
+        Wn = (W * (np.float32(1) / np.float32(nw) if sim.single else 1.0 / nw)).astype(cdt)
     while state.has_next:
         wig = sim.wiggles[(state.it - 1) % nw]
-        if warm and state.it == 1:
-            Etmp = propagate(sim, src.add_to(dE.copy(), A_warm), wig, workers)
-            inc = f_mix(dE, Etmp, g4, sim.single) - dE
-            E = E + dE + inc
-            dE = inc
+        if warm and state.it <= nw:
+            X = dE + Wn
+            Etmp = propagate(sim, src.add_to(X.copy(), A), wig, workers)
+            dE = f_mix(X, Etmp, g4, sim.single) - Wn
+            E = E + dE
         else:
-            if state.it <= nw and not warm:
+            if state.it <= nw:
                 Etmp = src.add_to(dE.copy(), A)
             else:
                 Etmp = dE

@@ -247,3 +247,17 @@ def test_warm_start_matches_oracle(gpu, builder):
     # back to the reference start
     Eg2, sg2 = g.exec(cases.gpu_source(src))
     assert sg2.iterations == sz.it - 1 and rel_l2(Eg2, Ez) <= TOL
+
+
+def test_set_medium_equals_new_solver(gpu):
+    """One solver reused for a second medium gives the second medium's results."""
+    import wavesim_amd as w
+    n1, opt, src = cases.case_2d_disordered()
+    n2 = cases.smooth_disorder(n1.shape, 99, absorption=0.02)
+    s1 = w.WaveSim(n1, opt)
+    s1.exec(cases.gpu_source(src))
+    s2 = w.WaveSim(n2, opt)
+    s2.adopt_solver(s1)
+    E2, st2 = s2.exec(cases.gpu_source(src))
+    E2n, st2n = w.WaveSim(n2, opt).exec(cases.gpu_source(src))
+    assert st2.iterations == st2n.iterations and np.array_equal(E2, E2n)

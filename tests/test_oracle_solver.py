@@ -181,13 +181,14 @@ def test_source_outside_grid_is_cropped():
     assert s.positions == []
 
 
-def test_warm_start_zero_equals_zero_start_nw1():
-    n, opt, src = cases.case_periodic_2d(thr=1e-8)
-    sim = mbs.make_sim(n, dict(opt, single_precision=False))
-    assert len(sim.wiggles) == 1
-    E1, s1 = mbs.run(sim, cases.oracle_source(src))
-    E2, s2 = mbs.run(sim, cases.oracle_source(src), E0=np.zeros(sim.N))
-    assert s1.it == s2.it and rel_l2(E2, E1) < 1e-13
+def test_warm_start_zero_equals_zero_start():
+    """E0 = 0 is exactly the reference loop, with and without wiggles."""
+    for builder in (cases.case_periodic_2d, cases.case_2d_disordered):
+        n, opt, src = builder()
+        sim = mbs.make_sim(n, dict(opt, single_precision=False))
+        E1, s1 = mbs.run(sim, cases.oracle_source(src))
+        E2, s2 = mbs.run(sim, cases.oracle_source(src), E0=np.zeros(sim.N))
+        assert s1.it == s2.it and rel_l2(E2, E1) < 1e-14
 
 
 def test_warm_start_from_solution_stops_at_once_nw1():
@@ -201,19 +202,21 @@ def test_warm_start_from_solution_stops_at_once_nw1():
     assert rel_l2(Ew, E) < 1e-6
 
 
-def test_warm_start_with_wiggles_close_to_zero_start():
-    """With Nw > 1 the zero-start answer is not the fixed point of one wiggle's
-    step, so a warm start converges to it only up to the wrap-around
-    correction (here 8-px boundaries: < 2%)."""
-    n, opt, src = cases.case_2d_disordered(thr=1e-12)
+def test_warm_start_with_wiggles_from_solution():
+    """With Nw > 1 the zero-start answer is not the fixed point of one
+    wiggle's step; spreading the correction over the wiggle cycle keeps a
+    warm start from the converged field close to it and quick."""
+    n, opt, src = cases.case_2d_disordered(thr=1e-14)
     sim = mbs.make_sim(n, dict(opt, single_precision=False))
     E, s = mbs.run(sim, cases.oracle_source(src))
     with np.errstate(all="ignore"):
         Efull = s.E_primed / sim.gamma[..., None]
     Efull[~np.isfinite(Efull)] = 0
-    sim2 = mbs.make_sim(n, dict(opt, single_precision=False, energy_threshold=1e-8))
+    sim2 = mbs.make_sim(n, dict(opt, single_precision=False, energy_threshold=1e-6))
+    _, sz = mbs.run(sim2, cases.oracle_source(src))
     Ew, sw = mbs.run(sim2, cases.oracle_source(src), E0=Efull, ref_energy=s.diff_energy[0])
-    assert sw.converged and rel_l2(Ew, E) < 2e-2
+    print("zero", sz.it - 1, "warm", sw.it - 1, "err", rel_l2(Ew, E))
+    assert sw.converged and sw.it < sz.it and rel_l2(Ew, E) < 2e-3
 
 
 def test_warm_start_converges_to_fixed_point_nw1():

@@ -67,6 +67,7 @@ SYMBOLS = {
     "wsim_solver_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "wsim_solver_get_plan": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "wsim_solver_set_plan": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "wsim_solver_set_medium": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, cfloat, C.POINTER(wiggle_t)]),
     "wsim_solver_set_warm_start": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "wsim_solver_clear_warm_start": (C.c_int, [C.c_void_p]),
     "wsim_solver_set_reference_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),

@@ -439,6 +439,7 @@ struct Solver {
       Launch L = make_launch(true, MODE_MID, PRO_SRC_RAMP, epi, axes[0], "pass_mid_pro_epi");
       L.args.in = in0;
       L.args.S = probe_mode ? nullptr : S;
+      L.args.W0 = probe_mode ? nullptr : W0;
       L.args.out = probe_mode ? probe : T;
       L.args.dE = dE;
       L.args.E = E;
@@ -449,6 +450,7 @@ struct Solver {
       Launch L0 = make_launch(true, MODE_FWD, PRO_SRC_RAMP, EPI_NONE, axes[0], "pass_fwd_pro");
       L0.args.in = in0;
       L0.args.S = probe_mode ? nullptr : S;
+      L0.args.W0 = probe_mode ? nullptr : W0;
       L0.args.out = T;
       out.push_back(L0);
       for (int k = 1; k < D - 1; ++k) {
@@ -474,6 +476,7 @@ struct Solver {
       LE.args.E = E;
       LE.args.gamma = gamma;
       LE.args.partials = partials;
+      LE.args.W0 = probe_mode ? nullptr : W0;
       out.push_back(LE);
     }
     if (probe_mode)
@@ -486,6 +489,39 @@ struct Solver {
   void launch(const Launch& L) {
     hipLaunchKernelGGL(L.fn, L.grid, dim3(NT), L.lds, stream, L.dargs);
     HIPCHK(hipGetLastError());
+  }
+
+  // per-wiggle coordinate and ramp tables (WaveSimBase.m:300-316)
+  void upload_wiggles(const wsim_wiggle* wig) {
+    for (int a = 0; a < 3; ++a) {
+      std::vector<float> pe((size_t)nw * N[a]);
+      std::vector<float2> rp((size_t)nw * N[a]);
+      for (int w = 0; w < nw; ++w) {
+        const float* src = a == 0 ? wig[w].pye : (a == 1 ? wig[w].pxe : wig[w].pze);
+        const wsim_cfloat* gr = a == 0 ? wig[w].gy : (a == 1 ? wig[w].gx : wig[w].gz);
+        require(src != nullptr && gr != nullptr, "null wiggle table");
+        for (int i = 0; i < N[a]; ++i) {
+          pe[(size_t)w * N[a] + i] = src[i];
+          rp[(size_t)w * N[a] + i] = make_float2(gr[i].re, gr[i].im);
+        }
+      }
+      HIPCHK(hipMemcpy(d_pe[a], pe.data(), pe.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_ramp[a], rp.data(), rp.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+  }
+
+  // a new medium on the same grid: gamma, epsilon, k02e and wiggle tables
+  // (sources are scaled by i/eps/Nw when set: set them again afterwards)
+  void set_medium(const wsim_cfloat* g, float eps, wsim_cfloat k, const wsim_wiggle* wig) {
+    require(g != nullptr && wig != nullptr, "null argument");
+    upload_wiggles(wig);
+    HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
+    epsilon = eps;
+    k02e = make_float2(k.re, k.im);
+    rebuild_launches();
+    std::fill(has_source.begin(), has_source.end(), 0);
+    const size_t fe = (size_t)field_elems() * nrhs;
+    HIPCHK(hipMemset(S, 0, fe * sizeof(float2)));
   }
 
   // twiddle tables of axis a for its current radix plan
@@ -732,22 +768,10 @@ struct Solver {
               "axis length " + std::to_string(N[a]) + " has a prime factor above 11",
               WSIM_ERR_UNSUPPORTED);
       build_tables(a);
-      std::vector<float> pe((size_t)nw * N[a]);
-      std::vector<float2> rp((size_t)nw * N[a]);
-      for (int w = 0; w < nw; ++w) {
-        const float* src = a == 0 ? wig[w].pye : (a == 1 ? wig[w].pxe : wig[w].pze);
-        const wsim_cfloat* gr = a == 0 ? wig[w].gy : (a == 1 ? wig[w].gx : wig[w].gz);
-        require(src != nullptr && gr != nullptr, "null wiggle table");
-        for (int i = 0; i < N[a]; ++i) {
-          pe[(size_t)w * N[a] + i] = src[i];
-          rp[(size_t)w * N[a] + i] = make_float2(gr[i].re, gr[i].im);
-        }
-      }
-      dalloc(&d_pe[a], pe.size());
-      dalloc(&d_ramp[a], rp.size());
-      HIPCHK(hipMemcpy(d_pe[a], pe.data(), pe.size() * sizeof(float), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_ramp[a], rp.data(), rp.size() * sizeof(float2), hipMemcpyHostToDevice));
+      dalloc(&d_pe[a], (size_t)nw * N[a]);
+      dalloc(&d_ramp[a], (size_t)nw * N[a]);
     }
+    upload_wiggles(wig);
     // fields
     const size_t fe = (size_t)field_elems() * nrhs;
     dalloc(&gamma, nvox);
@@ -827,7 +851,11 @@ struct Solver {
   void set_warm(int rhs, const wsim_cfloat* E0, bool on_device) {
     require(rhs >= 0 && rhs < nrhs && E0 != nullptr, "bad argument");
     const size_t f1 = (size_t)field_elems();
-    if (!W0) dalloc(&W0, f1 * nrhs);
+    if (!W0) {
+      dalloc(&W0, f1 * nrhs);
+      HIPCHK(hipMemset(W0, 0, f1 * nrhs * sizeof(float2)));
+      rebuild_launches();  // the passes take the W0 pointer
+    }
     float2* dst = W0 + rhs * f1;
     HIPCHK(hipMemcpy(dst, E0, f1 * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
     const unsigned blocks = (unsigned)std::min<size_t>((f1 + 255) / 256, 1 << 20);
@@ -841,6 +869,7 @@ struct Solver {
   void clear_warm() {
     warm_mode = false;
     std::fill(has_warm.begin(), has_warm.end(), 0);
+    if (W0) HIPCHK(hipMemset(W0, 0, (size_t)field_elems() * nrhs * sizeof(float2)));
   }
 
   void set_reference_energy(int rhs, double e) {
@@ -879,8 +908,6 @@ struct Solver {
     h.energy_all = d_eall;
     h.ref_energy = d_ref;
     h.warm = warm_mode ? 1 : 0;
-    h.src_iters = warm_mode ? 1 : nw;
-    h.src_scale = warm_mode ? (float)nw : 1.f;
     std::vector<int> st(nrhs), zero(nrhs, 0);
     int all = 1;
     for (int r = 0; r < nrhs; ++r) {
@@ -895,18 +922,12 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(d_niter, zero.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_last, inf.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, stream));
     const size_t fe = (size_t)field_elems() * nrhs;
-    HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
+    HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
     if (warm_mode) {
-      // dE' starts at gamma E0 for warm-started rhs, 0 for the others
-      const size_t f1 = (size_t)field_elems();
-      for (int r = 0; r < nrhs; ++r) {
-        if (has_warm[r])
-          HIPCHK(hipMemcpyAsync(dE + r * f1, W0 + r * f1, f1 * sizeof(float2), hipMemcpyDeviceToDevice, stream));
-        else
-          HIPCHK(hipMemsetAsync(dE + r * f1, 0, f1 * sizeof(float2), stream));
-      }
+      // E' starts at gamma E0 (W0 is zero for rhs without a warm start)
+      HIPCHK(hipMemcpyAsync(E, W0, fe * sizeof(float2), hipMemcpyDeviceToDevice, stream));
     } else {
-      HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
+      HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
     }
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -1223,6 +1244,15 @@ int wsim_solver_set_plan(wsim_solver* s, const char* plan) {
   return guard([&]() -> int {
     require(s != nullptr && plan != nullptr, "bad argument");
     s->s.set_plan(plan);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_medium(wsim_solver* s, const wsim_cfloat* gamma, float epsilon, wsim_cfloat k02e,
+                           const wsim_wiggle* wiggles) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.set_medium(gamma, epsilon, k02e, wiggles);
     return WSIM_OK;
   });
 }

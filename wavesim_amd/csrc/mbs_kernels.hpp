@@ -74,9 +74,7 @@ struct DevState {
   int nblk;         // energy partials per rhs
   long long max_iter;
   double threshold;
-  int src_iters;      // iterations that add the source: Nw (zero start) or 1 (warm start)
-  float src_scale;    // multiplier of the stored i/eps/Nw * S: 1, or Nw for a warm start
-  int warm;           // iteration 1 is the first step from a warm start E0 (dE' = gamma E0)
+  int warm;           // warm start: W0 = gamma E0 is spread over the first Nw iterations
   int* stopped;        // [nrhs]
   int* converged;      // [nrhs]
   int* n_iter;         // [nrhs]
@@ -110,6 +108,7 @@ struct PassArgs {
   const float2* in;
   float2* out;
   const float2* S;  // premultiplied dense source (i/eps/Nw * S), or null
+  const float2* W0; // warm start gamma*E0 per rhs, or null
   float2* dE;
   float2* E;
   const float2* gamma;
@@ -156,14 +155,15 @@ struct Pass {
   const ColInfo* cols;
   int tid;
   bool add_src;
-  float src_scale;
-  bool warm_step;  // first step from a warm start
+  bool warm_step;  // one of the first Nw steps of a warm start
+  float inv_nw;
   uint32_t lg_ncol;
   const float2* ramp_a;  // this wiggle's ramp along the axis
   const float* pe_a;
   const float2* in_b[C];  // workgroup bases, per component
   float2* out_b[C];
   const float2* S_b[C];
+  const float2* W0_b[C];
   float2* dE_b[C];
   float2* E_b[C];
   const float2* gam_b;
@@ -196,7 +196,8 @@ struct Pass {
     const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
     float2 v = ldg(in_b[c], i);
     if (PRO == PRO_SRC_RAMP) {
-      if (add_src) v = cadd(v, cscale(ldg(S_b[c], i), src_scale));
+      if (warm_step) v = cadd(v, cscale(ldg(W0_b[c], i), inv_nw));
+      if (add_src) v = cadd(v, ldg(S_b[c], i));
       v = cmul(v, cmul(ramp_a[row], ci.rc));
     }
     return v;
@@ -326,14 +327,18 @@ struct Pass {
 #pragma unroll
               for (int c = 0; c < C; ++c) {
                 const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
-                const float2 d0 = ldg(dE_b[c], i);
+                float2 d0 = ldg(dE_b[c], i);
                 const float2 e0 = ldg(E_b[c], i);
-                // warm start: dE'_1 = -a g (g E0) - 0.7 i g^2 Eprop and
-                // E' = g E0 + dE'_1, with dE' holding g E0 on entry
-                const float2 dn = warm_step ? csub(cmul(make_float2(c1.x - 1.f, c1.y), d0), cmul(c2, ep))
-                                            : csub(cmul(c1, d0), cmul(c2, ep));
+                // warm start (first Nw steps): the increment of the correction
+                // W0/Nw, i.e. (M_w - 1) W0/Nw, joins the recurrence
+                float2 w = make_float2(0.f, 0.f);
+                if (warm_step) {
+                  w = cscale(ldg(W0_b[c], i), inv_nw);
+                  d0 = cadd(d0, w);
+                }
+                const float2 dn = csub(csub(cmul(c1, d0), cmul(c2, ep)), w);
                 stg(dE_b[c], i, dn);
-                stg(E_b[c], i, warm_step ? cadd(cadd(e0, d0), dn) : cadd(e0, dn));
+                stg(E_b[c], i, cadd(e0, dn));
                 if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
                   eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
               }
@@ -499,8 +504,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   __syncthreads();
 
   using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT>;
-  P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= st->src_iters, st->src_scale,
-      st->warm != 0 && it_cur == 1, 0u, ramp_l, pe_l};
+  P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw,
+      st->warm != 0 && a.W0 != nullptr && it_cur <= nw, 1.f / (float)nw, 0u, ramp_l, pe_l};
   p.eacc = 0.0;
   if (!CONTIG) {
     p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
@@ -513,6 +518,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     p.in_b[c] = a.in + cb;
     p.out_b[c] = a.out + cb;
     p.S_b[c] = a.S + cb;
+    p.W0_b[c] = a.W0 + cb;
     p.dE_b[c] = a.dE + cb;
     p.E_b[c] = a.E + cb;
   }

@@ -308,6 +308,19 @@ class WaveSimBase:
                 self._solver.plan = plan
         return self._solver
 
+    def adopt_solver(self, other: "WaveSimBase"):
+        """Take over another simulation's solver (same grid, wiggle count and
+        ROI), loading this medium into it instead of allocating a new one."""
+        sv = other._solver
+        if sv is None or list(sv.N) != list(self.N) or sv.nw != len(self.wiggles) \
+                or not np.array_equal(other.roi, self.roi):
+            raise ValueError("solver shape does not match this simulation")
+        sv.set_medium(self.gamma, self.wiggles, float(self.epsilon), complex(self.k02e))
+        other._solver = None
+        self._solver = sv
+        self._solver_nrhs = other._solver_nrhs
+        return sv
+
     def tune(self, nrhs: int = 1, reps: int = 4):
         """Autotune the FFT plan of this problem (radix order and tile width
         per axis) and remember it for later solvers of the same shape."""

@@ -125,6 +125,15 @@ class MBSSolver:
         capi.check(self._lib.wsim_solver_propagate(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
         return out
 
+    def set_medium(self, gamma, wiggles, epsilon: float, k02e: complex):
+        """Replace the medium (same grid and wiggle count); sources are cleared."""
+        g = capi.c64(np.asarray(gamma).reshape(self.N[:3]))
+        w = capi.Wiggles(wiggles)
+        capi.check(self._lib.wsim_solver_set_medium(self._h, capi.ptr(g), float(epsilon),
+                                                    capi.cfloat(float(np.real(k02e)), float(np.imag(k02e))),
+                                                    w.arr))
+        self._w = w
+
     # -- warm start -------------------------------------------------------------
     def set_warm_start(self, rhs: int, E0=None, device_ptr: Optional[int] = None):
         """Start rhs from the unprimed field E0 ([y,x,z,pol] host array) or
