@@ -49,6 +49,21 @@ int wfno_adam(float* p, const float* g, float* m, float* v, long long n, float l
  * roi_lo / roi_n = 0-based ROI origin and extent; E0 in the solver layout. */
 int wfno_compose_warm(void* E0, const float* A, const void* u, const int grid[3], const int roi_lo[3],
                       const int roi_n[3], int factor, const int foot[4], void* stream);
+/* Straight-ray prior on the ROI (refractive index n, [Lz][Ly][Lx], x fastest,
+ * roi_n = [Lz Ly Lx]): phi = k0 h (sum_{z'<z}(n - n_bg) + (n(z) - n_bg)/2),
+ * P = u(z) exp(i phi) (complex64, same layout), u = background field at the
+ * ROI's z planes. */
+int wfno_wkb_prior(const float* n, float* phase, void* P, const void* u, const int roi_n[3], float k0h,
+                   float n_bg, void* stream);
+/* Network input [6][L^3] on the grid pooled by `factor`: block means of
+ * n^2 - n_bg^2, cos phi, sin phi, then the normalised coordinates z, y, x. */
+int wfno_features(const float* n, const float* phase, float* X, const int roi_n[3], int factor, float n_bg,
+                  void* stream);
+/* E0 = P (1 + A) in the ROI (P from wfno_wkb_prior, A the envelope [2][Z][Y][X]
+ * at 1/factor resolution), u(z) on the source footprint elsewhere; grid,
+ * roi_lo and roi_n in [y x z] order, E0 in the solver layout. */
+int wfno_compose_prior(void* E0, const float* A, const void* P, const void* u, const int grid[3],
+                       const int roi_lo[3], const int roi_n[3], int factor, const int foot[4], void* stream);
 /* dst[i] (+)= sum_{c < nchunks} src[c*n + i]  (fixed order) */
 int wfno_reduce_chunks(const float* src, float* dst, int nchunks, long long n, int accumulate, void* stream);
 const char* wfno_last_error(void);

@@ -97,3 +97,40 @@ def test_adam_step_matches_formula(gpu):
     v = 0.001 * g * g
     ref = p0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
     assert np.allclose(p1, ref, rtol=1e-5, atol=1e-6)
+
+
+def test_warmstart_preprocessing_kernels(gpu):
+    """wfno_wkb_prior, wfno_features and wfno_compose_prior vs numpy."""
+    import ctypes as C
+    import torch
+    from oracle import warmstart as O
+    from wavesim_amd.learned import kernels as Kn
+    rng = np.random.default_rng(7)
+    nz = ny = nx = 16
+    n = (1.33 + 0.3 * rng.random((nz, ny, nx))).astype(np.float32)
+    u = (rng.standard_normal(nz) + 1j * rng.standard_normal(nz)).astype(np.complex64)
+    k0h = 2 * np.pi * 0.25
+    phase_o, P_o = O.wkb_prior(n, u, k0h, 1.33)
+    dn = torch.from_numpy(n.ravel()).cuda()
+    du = torch.from_numpy(u.view(np.float32).copy()).cuda()
+    ph = torch.empty(n.size, device="cuda")
+    P = torch.empty(2 * n.size, device="cuda")
+    shape = (C.c_int * 3)(nz, ny, nx)
+    Kn.check(Kn.lib().wfno_wkb_prior(Kn.ptr(dn), Kn.ptr(ph), Kn.ptr(P), Kn.ptr(du), shape, k0h, 1.33,
+                                     Kn.stream_ptr()))
+    assert np.allclose(ph.cpu().numpy().reshape(n.shape), phase_o, atol=1e-5)
+    assert rel_l2(P.cpu().numpy().view(np.complex64).reshape(n.shape), P_o) <= 1e-5
+    X = torch.empty(6 * 8 ** 3, device="cuda")
+    Kn.check(Kn.lib().wfno_features(Kn.ptr(dn), Kn.ptr(ph), Kn.ptr(X), shape, 2, 1.33, Kn.stream_ptr()))
+    assert rel_l2(X.cpu().numpy().reshape(6, 8, 8, 8), O.features(n, phase_o, 2, 1.33)) <= 1e-5
+    grid, lo, roi, foot = (24, 22, 20), (3, 4, 2), (16, 16, 16), (3, 19, 4, 20)
+    A = rng.standard_normal((2, 8, 8, 8)).astype(np.float32)
+    uf = (rng.standard_normal(grid[2]) + 1j * rng.standard_normal(grid[2])).astype(np.complex64)
+    E0 = torch.empty(2 * int(np.prod(grid)), device="cuda")
+    Kn.check(Kn.lib().wfno_compose_prior(
+        Kn.ptr(E0), Kn.ptr(torch.from_numpy(A.ravel()).cuda()), Kn.ptr(P),
+        Kn.ptr(torch.from_numpy(uf.view(np.float32).copy()).cuda()), (C.c_int * 3)(*grid), (C.c_int * 3)(*lo),
+        (C.c_int * 3)(*roi), 2, (C.c_int * 4)(*foot), Kn.stream_ptr()))
+    ref = O.compose(grid, lo, roi, 2, P.cpu().numpy().view(np.complex64).reshape(n.shape), A, uf, foot)
+    got = E0.cpu().numpy().view(np.complex64).reshape(grid, order="F")
+    assert rel_l2(got, ref) <= 1e-6

@@ -261,3 +261,30 @@ def test_set_medium_equals_new_solver(gpu):
     E2, st2 = s2.exec(cases.gpu_source(src))
     E2n, st2n = w.WaveSim(n2, opt).exec(cases.gpu_source(src))
     assert st2.iterations == st2n.iterations and np.array_equal(E2, E2n)
+
+
+def test_warm_start_device_pointer_equals_host(gpu):
+    """A warm-start field handed over as device memory (written by another
+    stream) gives the same solve as the same field from host memory."""
+    import torch
+    import wavesim_amd as w
+    from wavesim_amd.workloads import spheres_3d
+    nmap, opt, src = spheres_3d(96, 16, seed=4)
+    sim = w.WaveSim(nmap, opt)
+    sv = sim.solver(1)
+    p = sim._prepare(src)
+    sv.set_source(0, p.positions, p.values)
+    rng = np.random.default_rng(3)
+    E0 = (rng.standard_normal(sim.N) + 1j * rng.standard_normal(sim.N)).astype(np.complex64) * 1e-3
+    out = []
+    for dev in (False, True):
+        if dev:
+            t = torch.from_numpy(np.asfortranarray(E0).ravel(order="F").view(np.float32).copy()).cuda()
+            t.mul_(1.0)  # pending work on torch's stream
+            sv.set_warm_start(0, device_ptr=int(t.data_ptr()))
+        else:
+            sv.set_warm_start(0, E0)
+        sv.solve(16, 0.0)
+        out.append(sv.field(0))
+        sv.clear_warm_start()
+    assert np.array_equal(out[0], out[1])

@@ -214,6 +214,87 @@ __global__ void compose_warm_kernel(float2* E0, const float* A, const float2* u,
   }
 }
 
+// Straight-ray (WKB) prior on the ROI, one thread per (y, x) column:
+//   phi(z) = k0 h (sum_{z'<z} (n - n_bg) + (n(z) - n_bg)/2),  P = u(z) exp(i phi)
+// n, phase: [Lz][Ly][Lx] (x fastest); u: the background field at the ROI's z.
+__global__ void wkb_kernel(const float* n, float* phase, float2* P, const float2* u, int Lz, int Ly, int Lx,
+                           float k0h, float n_bg) {
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)Ly * Lx;
+  if (col >= plane) return;
+  float acc = 0.f;
+  for (int z = 0; z < Lz; ++z) {
+    const long long i = z * plane + col;
+    const float d = n[i] - n_bg;
+    const float ph = k0h * (acc + 0.5f * d);
+    acc += d;
+    phase[i] = ph;
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    const float2 uz = u[z];
+    P[i] = make_float2(uz.x * cs - uz.y * sn, uz.x * sn + uz.y * cs);
+  }
+}
+
+// network input on the pooled grid (f^3 blocks): [6][L^3] = contrast
+// n^2 - n_bg^2, cos phi, sin phi (block means) and the coordinates z, y, x
+__global__ void features_kernel(const float* n, const float* phase, float* X, int Lz, int Ly, int Lx, int f,
+                                float nbg2) {
+  const int Z = Lz / f, Y = Ly / f, Xn = Lx / f;
+  const long long V = (long long)Z * Y * Xn;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < V;
+       j += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(j % Xn), y = (int)((j / Xn) % Y), z = (int)(j / ((long long)Xn * Y));
+    float de = 0.f, c = 0.f, sn = 0.f;
+    for (int dz = 0; dz < f; ++dz)
+      for (int dy = 0; dy < f; ++dy)
+        for (int dx = 0; dx < f; ++dx) {
+          const long long i = ((long long)(z * f + dz) * Ly + (y * f + dy)) * Lx + (x * f + dx);
+          const float nv = n[i];
+          de += nv * nv - nbg2;
+          float s1, c1;
+          sincosf(phase[i], &s1, &c1);
+          c += c1;
+          sn += s1;
+        }
+    const float inv = 1.f / (float)(f * f * f);
+    X[j] = de * inv;
+    X[V + j] = c * inv;
+    X[2 * V + j] = sn * inv;
+    X[3 * V + j] = (z + 0.5f) / Z - 0.5f;
+    X[4 * V + j] = (y + 0.5f) / Y - 0.5f;
+    X[5 * V + j] = (x + 0.5f) / Xn - 0.5f;
+  }
+}
+
+// E0 on the solver grid ([z][x][y], y fastest): P (1 + A) in the ROI, with P
+// the ROI prior ([z][y][x]) and A the envelope at 1/f resolution ([2][Z][Y][X]);
+// u(z) on the source footprint outside the ROI, 0 elsewhere.
+__global__ void compose_prior_kernel(float2* E0, const float* A, const float2* P, const float2* u, int Ny, int Nx,
+                                     int Nz, int y0, int x0, int z0, int ny, int nx, int nz, int f, int fy0,
+                                     int fy1, int fx0, int fx1) {
+  const long long total = (long long)Ny * Nx * Nz;
+  const int Xl = nx / f, Yl = ny / f, Zl = nz / f;
+  const long long plane = (long long)Xl * Yl * Zl;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(i % Ny);
+    const int x = (int)((i / Ny) % Nx);
+    const int z = (int)(i / ((long long)Ny * Nx));
+    float2 v = make_float2(0.f, 0.f);
+    const int ly = y - y0, lx = x - x0, lz = z - z0;
+    if (ly >= 0 && ly < ny && lx >= 0 && lx < nx && lz >= 0 && lz < nz) {
+      const float2 p = P[((long long)lz * ny + ly) * nx + lx];
+      const long long j = ((long long)(lz / f) * Yl + ly / f) * Xl + lx / f;
+      const float ar = 1.f + A[j], ai = A[plane + j];
+      v = make_float2(p.x * ar - p.y * ai, p.x * ai + p.y * ar);
+    } else if (y >= fy0 && y < fy1 && x >= fx0 && x < fx1) {
+      v = u[z];
+    }
+    E0[i] = v;
+  }
+}
+
 static unsigned grid_for(long long n) {
   long long b = (n + 255) / 256;
   return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
@@ -320,6 +401,46 @@ int wfno_compose_warm(void* E0, const float* A, const void* u, const int grid[3]
     hipLaunchKernelGGL(compose_warm_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)E0, A,
                        (const float2*)u, grid[0], grid[1], grid[2], roi_lo[0], roi_lo[1], roi_lo[2], roi_n[0],
                        roi_n[1], roi_n[2], factor, foot[0], foot[1], foot[2], foot[3]);
+    return 0;
+  });
+}
+
+int wfno_wkb_prior(const float* n, float* phase, void* P, const void* u, const int roi_n[3], float k0h,
+                   float n_bg, void* stream) {
+  return guard([&]() -> int {
+    const long long cols = (long long)roi_n[1] * roi_n[2];
+    hipLaunchKernelGGL(wkb_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, phase,
+                       (float2*)P, (const float2*)u, roi_n[0], roi_n[1], roi_n[2], k0h, n_bg);
+    return 0;
+  });
+}
+
+int wfno_features(const float* n, const float* phase, float* X, const int roi_n[3], int factor, float n_bg,
+                  void* stream) {
+  return guard([&]() -> int {
+    if (factor < 1 || roi_n[0] % factor || roi_n[1] % factor || roi_n[2] % factor) {
+      g_err = "ROI not divisible by the pooling factor";
+      return -1;
+    }
+    const long long V = (long long)(roi_n[0] / factor) * (roi_n[1] / factor) * (roi_n[2] / factor);
+    hipLaunchKernelGGL(features_kernel, dim3(grid_for(V)), dim3(256), 0, (hipStream_t)stream, n, phase, X,
+                       roi_n[0], roi_n[1], roi_n[2], factor, n_bg * n_bg);
+    return 0;
+  });
+}
+
+int wfno_compose_prior(void* E0, const float* A, const void* P, const void* u, const int grid[3],
+                       const int roi_lo[3], const int roi_n[3], int factor, const int foot[4], void* stream) {
+  return guard([&]() -> int {
+    if (factor < 1 || roi_n[0] % factor || roi_n[1] % factor || roi_n[2] % factor) {
+      g_err = "ROI not divisible by the network's down-sampling factor";
+      return -1;
+    }
+    const long long n = (long long)grid[0] * grid[1] * grid[2];
+    // roi_lo / roi_n are [y x z] like grid; P is [z][y][x] over the ROI
+    hipLaunchKernelGGL(compose_prior_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (float2*)E0, A,
+                       (const float2*)P, (const float2*)u, grid[0], grid[1], grid[2], roi_lo[0], roi_lo[1], roi_lo[2],
+                       roi_n[0], roi_n[1], roi_n[2], factor, foot[0], foot[1], foot[2], foot[3]);
     return 0;
   });
 }

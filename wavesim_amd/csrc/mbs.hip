@@ -522,6 +522,7 @@ struct Solver {
     std::fill(has_source.begin(), has_source.end(), 0);
     const size_t fe = (size_t)field_elems() * nrhs;
     HIPCHK(hipMemset(S, 0, fe * sizeof(float2)));
+    HIPCHK(hipDeviceSynchronize());  // uploads on the null stream are done before the solver stream runs
   }
 
   // twiddle tables of axis a for its current radix plan
@@ -806,6 +807,7 @@ struct Solver {
     ps.nrhs = 1;
     ps.stopped = p_stopped;
     HIPCHK(hipMemcpy(dprobe, &ps, sizeof(ps), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
   }
 
   void set_source(int rhs, int np, const wsim_source_patch* patches) {
@@ -844,6 +846,7 @@ struct Solver {
             }
     }
     HIPCHK(hipMemcpy(S + (size_t)rhs * fe, h.data(), fe * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
     has_source[rhs] = any ? 1 : 0;
   }
 
@@ -857,7 +860,12 @@ struct Solver {
       rebuild_launches();  // the passes take the W0 pointer
     }
     float2* dst = W0 + rhs * f1;
-    HIPCHK(hipMemcpy(dst, E0, f1 * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    // E0 may come from any stream: wait for all device work, then copy on
+    // the solver's stream (a device-to-device hipMemcpy would return before
+    // the copy is done and is not ordered with that stream)
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpyAsync(dst, E0, f1 * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          stream));
     const unsigned blocks = (unsigned)std::min<size_t>((f1 + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(gamma_mul_kernel, dim3(blocks), dim3(256), 0, stream, dst, gamma, (long long)nvox, C);
     HIPCHK(hipGetLastError());
@@ -870,6 +878,7 @@ struct Solver {
     warm_mode = false;
     std::fill(has_warm.begin(), has_warm.end(), 0);
     if (W0) HIPCHK(hipMemset(W0, 0, (size_t)field_elems() * nrhs * sizeof(float2)));
+    HIPCHK(hipDeviceSynchronize());
   }
 
   void set_reference_energy(int rhs, double e) {
@@ -1021,6 +1030,7 @@ struct Solver {
     const size_t fe = (size_t)field_elems();
     if (!probe) dalloc(&probe, fe);
     HIPCHK(hipMemcpy(probe, in, fe * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
     DevState ps{};
     ps.it = w + 1 + nw;  // wiggle w, past the source iterations
     ps.nw = nw;

@@ -43,7 +43,7 @@ def main(argv=None):
     if a.load:
         ws = WarmStart.load(a.load)
     else:
-        ws = WarmStart(Config(L=L, cin=4, width=a.width, modes=a.modes, layers=a.layers, proj=64),
+        ws = WarmStart(Config(L=L, cin=6, width=a.width, modes=a.modes, layers=a.layers, proj=64),
                        factor=a.factor)
     log(f"network: {ws.net.nparam} parameters, grid {L}^3")
     summary = {"parameters": ws.net.nparam, "grid": L}

@@ -1,15 +1,18 @@
 """Learned warm start for plane-wave problems (no counterpart in the reference).
 
-The network predicts the envelope A of the field relative to the closed-form
-background solution u(z) of the source plane (``prior.green_1d``):
-    E0 = u(z) (1 + A)  inside the ROI,   u(z) on the source footprint outside.
-Input features on a grid pooled by ``factor`` from the ROI: the permittivity
-contrast n^2 - n_bg^2 and the three normalised coordinates.  The network runs
-on the project's HIP kernels (``fno.FNO``), the field is composed on the
-solver grid by ``wfno_compose_warm`` and handed to the MBS solver as a warm
-start (``wsim_solver_set_warm_start``); the plain MBS iteration then runs to
-the same absolute residual as a zero start, so zero-start MBS is both the
-fallback and the judge of the warm start's error.
+Prior: the straight-ray (WKB) field of the source plane, computed at full
+resolution on the GPU (``wfno_wkb_prior``): the closed-form background field
+u(z) of the plane (``prior.green_1d``, the reference utility
+homogeneous_medium_analytic_solution.m:26-35) times the phase
+exp(i k0 int (n - n_bg) dz) accumulated along each ray.  The network predicts
+the envelope A of the field relative to that prior,
+    E0 = P (1 + A)  inside the ROI,   u(z) on the source footprint outside,
+from features on a grid pooled by ``factor`` (``wfno_features``): contrast
+n^2 - n_bg^2, cos/sin of the ray phase and the normalised coordinates.  The
+network runs on the project's HIP kernels (``fno.FNO``); E0 is handed to the
+MBS solver as a warm start (``wsim_solver_set_warm_start``) and the plain MBS
+iteration runs to the same absolute residual as a zero start, so zero-start
+MBS is both the fallback and the judge of the warm start's error.
 
 Training data are generated by the solver itself (zero-start solves of
 seeded sphere media, ``workloads.spheres_3d``).
@@ -17,8 +20,8 @@ seeded sphere media, ``workloads.spheres_3d``).
 from __future__ import annotations
 
 import concurrent.futures as cf
+import ctypes as C
 import time
-from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -27,22 +30,18 @@ from . import kernels as K
 from .fno import FNO, Config
 from .prior import green_1d
 
-
-@dataclass
-class Sample:
-    x: np.ndarray        # [cin][L^3] features
-    t: np.ndarray        # [2][L^3] target envelope
-    iterations: int
-    seconds: float
+CIN = 6
 
 
 class WarmStart:
     def __init__(self, cfg: Config | None = None, factor: int = 2, n_bg: float = 1.33, params=None,
                  seed: int = 0):
-        self.cfg = cfg or Config()
+        self.cfg = cfg or Config(cin=CIN)
+        assert self.cfg.cin == CIN
         self.factor = factor
         self.n_bg = n_bg
         self.net = FNO(self.cfg, params, seed=seed)
+        self.dev = self.net.dev
 
     # -- geometry ------------------------------------------------------------------
     @staticmethod
@@ -56,48 +55,57 @@ class WarmStart:
         k = sim.k0 * self.n_bg
         return green_1d(k, sim.grid.dx, (np.arange(sim.N[2]) - zs) * sim.grid.dx).astype(np.complex64)
 
+    def prepare(self, sim, nmap_roi) -> dict:
+        """Device-side prior and network input from the ROI refractive index
+        (nmap_roi in [y][x][z] order, as the reference stores media)."""
+        sl = self.roi_slices(sim)
+        nz, ny, nx = (sl[2].stop - sl[2].start, sl[0].stop - sl[0].start, sl[1].stop - sl[1].start)
+        n = torch.from_numpy(np.ascontiguousarray(np.real(nmap_roi).astype(np.float32).transpose(2, 0, 1)))
+        n = n.to(self.dev).reshape(-1)
+        u_full = self.prior_u(sim)
+        u = torch.from_numpy(u_full.view(np.float32).copy()).to(self.dev)
+        u_roi = torch.from_numpy(u_full[sl[2]].view(np.float32).copy()).to(self.dev)
+        phase = torch.empty(nz * ny * nx, dtype=torch.float32, device=self.dev)
+        P = torch.empty(2 * nz * ny * nx, dtype=torch.float32, device=self.dev)
+        shape = (C.c_int * 3)(nz, ny, nx)
+        K.check(K.lib().wfno_wkb_prior(K.ptr(n), K.ptr(phase), K.ptr(P), K.ptr(u_roi), shape,
+                                       float(sim.k0 * sim.grid.dx), float(self.n_bg), K.stream_ptr()))
+        f = self.factor
+        V = (nz // f) * (ny // f) * (nx // f)
+        assert nz // f == self.cfg.L and ny // f == self.cfg.L and nx // f == self.cfg.L, "grid mismatch"
+        X = torch.empty(CIN * V, dtype=torch.float32, device=self.dev)
+        K.check(K.lib().wfno_features(K.ptr(n), K.ptr(phase), K.ptr(X), shape, f, float(self.n_bg),
+                                      K.stream_ptr()))
+        return dict(X=X, P=P, u=u, shape=(nz, ny, nx))
+
     def _pool(self, a):
         f = self.factor
         Z, Y, X = a.shape
         return a.reshape(Z // f, f, Y // f, f, X // f, f).mean(axis=(1, 3, 5))
 
-    def features(self, nmap_roi) -> np.ndarray:
-        """nmap_roi: [y][x][z] refractive index of the ROI -> [cin][L^3]."""
-        d = (np.real(nmap_roi).astype(np.float32) ** 2 - np.float32(self.n_bg ** 2)).transpose(2, 0, 1)
-        d = self._pool(d)
-        L = d.shape[0]
-        assert d.shape == (L, L, L) and L == self.cfg.L, (d.shape, self.cfg.L)
-        c = (np.arange(L, dtype=np.float32) + 0.5) / L - 0.5
-        cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
-        return np.stack([d, cz, cy, cx]).reshape(4, -1).astype(np.float32)
-
-    def target(self, sim, E_roi) -> np.ndarray:
-        """Envelope of the solved field E_roi ([y][x][z] complex, ROI) relative to u."""
-        u = self.prior_u(sim)[self.roi_slices(sim)[2]]
-        A = (E_roi / u[None, None, :] - 1).transpose(2, 0, 1)
+    def target(self, sim, E_roi, prep) -> np.ndarray:
+        """Envelope of the solved field E_roi ([y][x][z] complex, ROI) relative to the prior."""
+        nz, ny, nx = prep["shape"]
+        P = prep["P"].cpu().numpy().view(np.complex64).reshape(nz, ny, nx)
+        A = E_roi.transpose(2, 0, 1) / P - 1
         A = self._pool(A)
         return np.stack([A.real, A.imag]).reshape(2, -1).astype(np.float32)
 
     # -- inference -----------------------------------------------------------------
-    def predict_E0(self, sim, x: np.ndarray | None = None, nmap_roi=None) -> torch.Tensor:
+    def predict_E0(self, sim, prep) -> torch.Tensor:
         """Warm-start field on the solver grid (device, solver layout)."""
-        if x is None:
-            x = self.features(nmap_roi)
-        xt = torch.from_numpy(x).to(self.net.dev).reshape(1, self.cfg.cin, -1)
-        y, _ = self.net.forward(xt, keep=False)
-        return self.compose(sim, y)
+        y, _ = self.net.forward(prep["X"].reshape(1, CIN, -1), keep=False)
+        return self.compose(sim, prep, y)
 
-    def compose(self, sim, y: torch.Tensor) -> torch.Tensor:
-        import ctypes as C
-        u = torch.from_numpy(self.prior_u(sim).view(np.float32).copy()).to(self.net.dev)
-        E0 = torch.empty(2 * int(np.prod(sim.N[:3])), dtype=torch.float32, device=self.net.dev)
+    def compose(self, sim, prep, y: torch.Tensor) -> torch.Tensor:
+        E0 = torch.empty(2 * int(np.prod(sim.N[:3])), dtype=torch.float32, device=self.dev)
         sl = self.roi_slices(sim)
         grid = (C.c_int * 3)(*sim.N[:3])
         lo = (C.c_int * 3)(*[s.start for s in sl])
         n = (C.c_int * 3)(*[s.stop - s.start for s in sl])
         foot = (C.c_int * 4)(sl[0].start, sl[0].stop, sl[1].start, sl[1].stop)
-        K.check(K.lib().wfno_compose_warm(K.ptr(E0), K.ptr(y), K.ptr(u), grid, lo, n, self.factor, foot,
-                                          K.stream_ptr()))
+        K.check(K.lib().wfno_compose_prior(K.ptr(E0), K.ptr(y), K.ptr(prep["P"]), K.ptr(prep["u"]), grid, lo, n,
+                                           self.factor, foot, K.stream_ptr()))
         return E0
 
     # -- training ------------------------------------------------------------------
@@ -186,8 +194,9 @@ def generate(ws: WarmStart, seeds, n=128, boundary=32, log=None):
         r = sv.result(0)
         with np.errstate(all="ignore"):
             E = sim.crop_field(sv.field(0) / sim.gamma[..., None])[..., 0]
-        X.append(ws.features(nmap))
-        T.append(ws.target(sim, E))
+        prep = ws.prepare(sim, nmap)
+        X.append(prep["X"].cpu().numpy().reshape(CIN, -1))
+        T.append(ws.target(sim, E, prep))
         info.append((seed, r.iterations, dt))
         if log and len(info) % 50 == 0:
             log(f"generated {len(info)} samples (last: {r.iterations} iterations, {dt * 1e3:.1f} ms)")
@@ -223,15 +232,15 @@ def evaluate(ws: WarmStart, seeds, holder=None, n=128, boundary=32, tight=1e-10)
         d1 = float(rz.diff_energy[0])
         err = lambda E: float(np.linalg.norm(E - Eref) / np.linalg.norm(Eref))
         out = dict(seed=int(seed), zero_its=rz.iterations, zero_s=t_zero, zero_err=err(Ez))
-        x = ws.features(nmap)
         for name in ("prior", "learned"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
+            prep = ws.prepare(sim, nmap)
             if name == "learned":
-                E0 = ws.predict_E0(sim, x=x)
+                E0 = ws.predict_E0(sim, prep)
             else:
                 zero = torch.zeros(2 * ws.cfg.L ** 3, dtype=torch.float32, device=ws.net.dev)
-                E0 = ws.compose(sim, zero)
+                E0 = ws.compose(sim, prep, zero)
             sv.set_warm_start(0, device_ptr=K.ptr(E0))
             sv.set_reference_energy(0, d1)
             t_pred = time.perf_counter() - t0
