@@ -10,6 +10,10 @@ timed on the solver's stream) and the CPU baseline (the numpy restatement of
 the reference loop on this host's cores, bounded sample, extrapolated).
 
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Counter collection (short runs for rocprofv3 --pmc):
+    python bench.py --trace-iterations 24     # 24 iterations of the workload, then exit
+    python bench.py --copy-calibration        # 1 GiB device-to-device copy, 3 times
 """
 from __future__ import annotations
 
@@ -73,6 +77,38 @@ def cpu_baseline(nmap, opt, source, gpu_iterations: int, seconds: float, vector=
                 ms_per_iteration=per_it * 1e3)
 
 
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """A plain copy of known size (1 GiB read + 1 GiB written per launch) whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the solver kernels."""
+    import torch
+    x = torch.ones(COPY_BYTES // 8, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copy calibration: 3 launches of {COPY_BYTES} bytes read + written", flush=True)
+
+
+def load_traffic(workload: str, plan: str):
+    """HBM bytes per launch of each pass measured with rocprofv3 --pmc and
+    scaled by the copy calibration (profiles/hbm_traffic_<workload>.txt, lines
+    'traffic <pass> <fetch_bytes> <write_bytes>'), if measured for this plan."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return {}
+    out, file_plan = {}, None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == "plan":
+            file_plan = f[1]
+        elif len(f) == 4 and f[0] == "traffic":
+            out[f[1]] = (float(f[2]), float(f[3]))
+    return out if file_plan == plan else {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,11 +119,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="keep the default FFT plan when none is cached")
+    ap.add_argument("--trace-iterations", type=int, default=0,
+                    help="run this many iterations of the workload and exit (counter collection)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run the 1 GiB copy whose counter readings calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the solver is single-GPU); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     from wavesim_amd import WaveSim, WaveSimVector, plans
@@ -107,6 +151,11 @@ def main():
     max_it = sim.max_iterations()
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f}s grid {sim.N} nw {len(sim.wiggles)}", file=sys.stderr, flush=True)
+    if args.trace_iterations > 0:
+        sv.solve(args.trace_iterations, 0.0)
+        torch.cuda.synchronize()
+        print(f"traced {sv.result(0).iterations} iterations, plan {sv.plan}", flush=True)
+        return
 
     for _ in range(args.warmup):
         sv.solve(max_it, sim.energy_threshold)
@@ -138,6 +187,8 @@ def main():
     print("# kernels: " + ", ".join(f"{n} {ms * 1e3:.1f}us {b / ms / 1e6:.0f}GB/s" if b else f"{n} {ms * 1e3:.1f}us"
                                    for n, ms, b in prof), file=sys.stderr, flush=True)
 
+    traffic = load_traffic(args.workload, sv.plan).get(dom[0])
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nmap, opt, source, its, args.cpu_seconds, vector=vector)
@@ -168,7 +219,9 @@ def main():
         "kernels_us": {n: round(ms * 1e3, 2) for n, ms, _ in prof},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None, "bytes_per_launch": dom[2], "launch_us": round(dom[1] * 1e3, 2)},
+                     "traffic": None if traffic is None else round(sum(traffic)),
+                     "traffic_fetch_write": traffic, "bytes_per_launch": dom[2],
+                     "launch_us": round(dom[1] * 1e3, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

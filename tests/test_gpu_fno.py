@@ -127,9 +127,10 @@ def test_warmstart_preprocessing_kernels(gpu):
     A = rng.standard_normal((2, 8, 8, 8)).astype(np.float32)
     uf = (rng.standard_normal(grid[2]) + 1j * rng.standard_normal(grid[2])).astype(np.complex64)
     E0 = torch.empty(2 * int(np.prod(grid)), device="cuda")
+    dA = torch.from_numpy(A.ravel()).cuda()      # kept alive: raw pointers below
+    duf = torch.from_numpy(uf.view(np.float32).copy()).cuda()
     Kn.check(Kn.lib().wfno_compose_prior(
-        Kn.ptr(E0), Kn.ptr(torch.from_numpy(A.ravel()).cuda()), Kn.ptr(P),
-        Kn.ptr(torch.from_numpy(uf.view(np.float32).copy()).cuda()), (C.c_int * 3)(*grid), (C.c_int * 3)(*lo),
+        Kn.ptr(E0), Kn.ptr(dA), Kn.ptr(P), Kn.ptr(duf), (C.c_int * 3)(*grid), (C.c_int * 3)(*lo),
         (C.c_int * 3)(*roi), 2, (C.c_int * 4)(*foot), Kn.stream_ptr()))
     ref = O.compose(grid, lo, roi, 2, P.cpu().numpy().view(np.complex64).reshape(n.shape), A, uf, foot)
     got = E0.cpu().numpy().view(np.complex64).reshape(grid, order="F")

@@ -1,0 +1,133 @@
+"""Summaries of rocprofv3 CSV output for the MBS pass kernels.
+
+    python -m wavesim_amd.rocprof_summary stats DIR
+        per-kernel dispatch count and average / median duration from the
+        kernel trace (rocprofv3 --kernel-trace --stats --output-format csv -d DIR)
+    python -m wavesim_amd.rocprof_summary traffic --fetch D1 --write D2 --copy-fetch D3 --copy-write D4
+        HBM bytes per launch of each pass from FETCH_SIZE / WRITE_SIZE runs
+        (rocprofv3 --pmc ... --output-format csv), each scaled by the reading of
+        the same counter for the 1 GiB copy of `bench.py --copy-calibration`
+
+The pass kernels are instantiations axis_pass_kernel<CONTIG, MODE, PRO, EPI,
+C, NT, EPT>; they are reported under the pass names the solver's profile()
+uses.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+import statistics
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 1 << 30
+
+_PASS = {  # (CONTIG, MODE, PRO, EPI) -> pass name
+    ("true", "0", "1", "0"): "pass_fwd_pro",
+    ("false", "0", "0", "0"): "pass_fwd",
+    ("false", "2", "0", "0"): "pass_mid_green",
+    ("false", "1", "0", "0"): "pass_inv",
+    ("true", "1", "0", "1"): "pass_inv_mix",
+    ("true", "1", "0", "2"): "pass_inv_mix",
+    ("true", "2", "1", "1"): "pass_mid_pro_epi",
+    ("true", "2", "1", "2"): "pass_mid_pro_epi",
+}
+
+
+def short_name(kernel: str) -> str:
+    m = re.search(r"axis_pass_kernel<\s*(\w+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)", kernel)
+    if m:
+        return _PASS.get(m.groups()[:4], "axis_pass") + f"[C={m.group(5)}]"
+    if "control_kernel" in kernel:
+        return "control"
+    if "copyBuffer" in kernel:
+        return "copyBuffer"
+    if "elementwise_kernel" in kernel and "copy" in kernel.lower():
+        return "torch_copy"
+    return kernel.split("(")[0][-60:]
+
+
+def _one(pattern: str, d: str) -> str:
+    files = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no {pattern} under {d}")
+    return files[0]
+
+
+def kernel_durations(d: str):
+    """{kernel: [duration ns, ...]} from *_kernel_trace.csv."""
+    out = defaultdict(list)
+    with open(_one("*kernel_trace.csv", d)) as f:
+        for r in csv.DictReader(f):
+            out[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    return out
+
+
+def counter_means(d: str, counter: str):
+    """{kernel: mean counter value per dispatch} from *_counter_collection.csv."""
+    acc = defaultdict(list)
+    with open(_one("*counter_collection.csv", d)) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def stats(d: str) -> str:
+    dur = kernel_durations(d)
+    total = sum(sum(v) for v in dur.values())
+    lines = [f"{'kernel':28s} {'calls':>6s} {'avg_us':>9s} {'median_us':>9s} {'min_us':>9s} {'time%':>6s}"]
+    for k, v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{short_name(k):28s} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} "
+                     f"{statistics.median(v) / 1e3:9.2f} {min(v) / 1e3:9.2f} {100 * sum(v) / total:6.2f}")
+    return "\n".join(lines)
+
+
+def _copy_reading(d: str, counter: str) -> float:
+    """Counter reading (KiB per dispatch) of the 1 GiB copy launches."""
+    best = 0.0
+    for k, (mean, n) in counter_means(d, counter).items():
+        best = max(best, mean)
+    return best
+
+
+def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str) -> str:
+    cf = _copy_reading(copy_fetch, "FETCH_SIZE")
+    cw = _copy_reading(copy_write, "WRITE_SIZE")
+    sf, sw = COPY_BYTES / cf, COPY_BYTES / cw  # bytes per KiB reading
+    lines = [f"# copy calibration: 1 GiB copy reads FETCH_SIZE {cf:.1f} KiB -> {sf:.1f} B per unit, "
+             f"WRITE_SIZE {cw:.1f} KiB -> {sw:.1f} B per unit"]
+    fm, wm = counter_means(fetch, "FETCH_SIZE"), counter_means(write, "WRITE_SIZE")
+    rows = {}
+    for k in fm:
+        if k in wm and "axis_pass_kernel" in k:
+            rows[short_name(k).split("[")[0]] = (fm[k][0] * sf, wm[k][0] * sw, fm[k][1])
+    for name, (fb, wb, n) in rows.items():
+        lines.append(f"traffic {name} {fb:.0f} {wb:.0f}")
+    lines.append(f"# {'pass':16s} {'fetch_MB':>10s} {'write_MB':>10s} {'dispatches':>10s}")
+    for name, (fb, wb, n) in rows.items():
+        lines.append(f"# {name:16s} {fb / 1e6:10.1f} {wb / 1e6:10.1f} {n:10d}")
+    return "\n".join(lines)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    t = sub.add_parser("traffic")
+    for name in ("fetch", "write", "copy-fetch", "copy-write"):
+        t.add_argument("--" + name, required=True)
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print(stats(a.dir))
+    else:
+        print(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
