@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="keep the default FFT plan when none is cached")
+    ap.add_argument("--plan", default="", help="FFT plan text to use instead of the cached/tuned one")
     ap.add_argument("--trace-iterations", type=int, default=0,
                     help="run this many iterations of the workload and exit (counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -143,7 +144,10 @@ def main():
     sim = (WaveSimVector if vector else WaveSim)(nmap, opt)
     sv = sim.solver(1)
     tuned = "shipped/cached"
-    if plans.lookup(sim.N, 1, len(sim.wiggles)) is None and not args.no_tune:
+    if args.plan:
+        sv.plan = args.plan
+        tuned = "given (--plan)"
+    elif plans.lookup(sim.N, 1, len(sim.wiggles)) is None and not args.no_tune:
         plan, _ = sim.tune(1, 4)
         tuned = "autotuned at setup"
     src = sim._prepare(source)

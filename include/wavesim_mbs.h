@@ -162,6 +162,23 @@ int wsim_solver_clear_warm_start(wsim_solver* s);
 int wsim_solver_set_reference_energy(wsim_solver* s, int rhs, double energy);
 int wsim_solver_destroy(wsim_solver* s);
 
+/* ---- groups of independent problems -------------------------------------
+ * Replaces a loop of exec calls over different simulations (the reference's
+ * way of running several problems: examples/pulse2d.m:31-40,
+ * examples/phase_conjugation.m:48-56) by one schedule: the solvers' (any
+ * grid sizes, sources, media) passes run as shared launches, each problem
+ * owning a range of workgroups, with one control launch for all problems per
+ * iteration.  Every problem keeps its own state, stop test and iteration
+ * count, and reads back through wsim_solver_result / wsim_solver_get_field
+ * exactly as after wsim_solver_solve (identical results).  The group
+ * references the solvers: destroy it before them.  Solvers must be on one
+ * device; a solver may belong to several groups but not twice to one. */
+typedef struct wsim_group wsim_group;
+int wsim_group_create(wsim_group** out, wsim_solver* const* solvers, int n);
+/* max_iter[i], energy_threshold[i]: as wsim_solver_solve, per solver */
+int wsim_group_solve(wsim_group* g, const int64_t* max_iter, const double* energy_threshold, int64_t batch);
+int wsim_group_destroy(wsim_group* g);
+
 const char* wsim_last_error(void);
 int wsim_abi_version(void);
 /* Host-side self checks of the index arithmetic the kernels rely on. */

@@ -6,9 +6,9 @@ RunWaveSimAlgorithmMex) runs on one GPU as hand-written gfx950 HIP kernels in
 host side: ``WaveSim`` / ``WaveSimVector`` / ``Source`` mirror the reference
 classes, ``MBSSolver`` wraps the handle API.
 """
-from .simulation import (Medium, SimulationGrid, Source, SimState, WaveSim, WaveSimBase,
+from .simulation import (Medium, SimulationGrid, Source, SimState, WaveSim, WaveSimBase, exec_many,
                          WaveSimVector, efficient_size, symrange)
-from .solver import MBSSolver, RhsResult
+from .solver import MBSSolver, RhsResult, SolverGroup
 
-__all__ = ["Medium", "SimulationGrid", "Source", "SimState", "WaveSim", "WaveSimBase",
+__all__ = ["Medium", "SimulationGrid", "Source", "SimState", "WaveSim", "WaveSimBase", "exec_many", "SolverGroup",
            "WaveSimVector", "efficient_size", "symrange", "MBSSolver", "RhsResult"]

@@ -38,7 +38,7 @@ def _flags():
     return ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument",
-            "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+            "-I", os.path.join(ROOT, "include"), "-I", CSRC] + os.environ.get("WSIM_HIPCC_FLAGS", "").split()
 
 
 def _run(cmd):
@@ -55,10 +55,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), [])]
     for ct, m, p, e in VARIANTS:
         for c in COMPONENTS:
-            name = f"v_{ct}_{m}_{p}_{e}_{c}"
-            units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
-                          [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",
-                           f"-DWSIM_EPI={e}", f"-DWSIM_C={c}"]))
+            for k in (0, 1):
+                name = f"v_{ct}_{m}_{p}_{e}_{c}_{k}"
+                units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
+                              [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",
+                               f"-DWSIM_EPI={e}", f"-DWSIM_C={c}", f"-DWSIM_COMPACT={k}"]))
     fno_obj = os.path.join(OBJ, "fno.o")
     jobs = jobs or min(16, os.cpu_count() or 4)
     cmds = [[hipcc()] + _flags() + defs + ["-c", src, "-o", obj] for src, obj, defs in units]

@@ -72,6 +72,9 @@ SYMBOLS = {
     "wsim_solver_clear_warm_start": (C.c_int, [C.c_void_p]),
     "wsim_solver_set_reference_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "wsim_solver_destroy": (C.c_int, [C.c_void_p]),
+    "wsim_group_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),
+    "wsim_group_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int64]),
+    "wsim_group_destroy": (C.c_int, [C.c_void_p]),
     "wsim_last_error": (C.c_char_p, []),
     "wsim_abi_version": (C.c_int, []),
     "wsim_selftest": (C.c_int, []),

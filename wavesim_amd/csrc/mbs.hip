@@ -154,21 +154,35 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
 
 // One block: reduce the energy partials of every rhs, record diff_energy the
 // way Simulation.next does (Simulation.m:193-217) and advance the iteration.
-__global__ void __launch_bounds__(256) control_kernel(DevState* st, const double* partials, int nblk) {
-  __shared__ double red[4];
+constexpr int kControlThreads = 1024;
+__device__ void control_one(DevState* st, const double* partials, int nblk) {
+  constexpr int NW = kControlThreads / 64;
+  __shared__ double red[NW];
   if (st->all_stopped) return;
   const int it = st->it;
   const int nrhs = st->nrhs;
   int all = 1;
   for (int r = 0; r < nrhs; ++r) {
     if (st->stopped[r]) continue;
-    double e = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += 256) e += partials[(size_t)r * nblk + i];
+    // four independent accumulators keep four loads in flight per thread;
+    // the summation order is fixed, so the result is deterministic
+    const double* pr = partials + (size_t)r * nblk;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * kControlThreads < nblk; i += 4 * kControlThreads) {
+      e0 += pr[i];
+      e1 += pr[i + kControlThreads];
+      e2 += pr[i + 2 * kControlThreads];
+      e3 += pr[i + 3 * kControlThreads];
+    }
+    for (; i < nblk; i += kControlThreads) e0 += pr[i];
+    double e = (e0 + e1) + (e2 + e3);
     for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
     if (threadIdx.x == 0) {
-      const double en = red[0] + red[1] + red[2] + red[3];
+      double en = 0.0;
+      for (int k = 0; k < NW; ++k) en += red[k];
       const size_t h = (size_t)r * st->hist_cap + (size_t)(it - 1);
       st->energy_all[h] = en;
       if (st->calc_energy) st->last_energy[r] = en;
@@ -197,6 +211,24 @@ __global__ void __launch_bounds__(256) control_kernel(DevState* st, const double
   }
 }
 
+__global__ void __launch_bounds__(kControlThreads) control_kernel(DevState* st, const double* partials, int nblk) {
+  control_one(st, partials, nblk);
+}
+
+// grouped launches: workgroup e runs the control step of problem e and
+// publishes its all-stopped flag into one array the host reads at once
+struct CtlEntry {
+  DevState* st;
+  const double* partials;
+  int nblk;
+  int* done;
+};
+__global__ void __launch_bounds__(kControlThreads) control_group_kernel(const CtlEntry* __restrict__ ce) {
+  const CtlEntry& c = ce[blockIdx.x];
+  control_one(c.st, c.partials, c.nblk);
+  if (threadIdx.x == 0) *c.done = c.st->all_stopped;
+}
+
 // x[c][v] *= gamma[v] for the C components of one field (warm-start seeding)
 __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2* g, long long nvox, int C) {
   const long long n = nvox * C;
@@ -207,14 +239,13 @@ __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2*
 // ----------------------------------------------------------------------------
 // kernel table
 // ----------------------------------------------------------------------------
-constexpr int NT = kThreads;
 constexpr size_t LDS_CAP = 80 * 1024;  // two workgroups per 160 KB CU
 
-static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C) {
-#define WSIM_SELECT(ct, m, p, e)                                           \
-  if ((contig ? 1 : 0) == ct && mode == m && pro == p && epi == e)         \
-    return C == 1 ? wsim_kptr_##ct##_##m##_##p##_##e##_1()                  \
-                  : wsim_kptr_##ct##_##m##_##p##_##e##_3();
+static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C, bool compact) {
+#define WSIM_SELECT(ct, m, p, e)                                                                       \
+  if ((contig ? 1 : 0) == ct && mode == m && pro == p && epi == e)                                     \
+    return C == 1 ? (compact ? wsim_kptr_##ct##_##m##_##p##_##e##_1_1() : wsim_kptr_##ct##_##m##_##p##_##e##_1_0()) \
+                  : (compact ? wsim_kptr_##ct##_##m##_##p##_##e##_3_1() : wsim_kptr_##ct##_##m##_##p##_##e##_3_0());
   WSIM_FOR_EACH_VARIANT(WSIM_SELECT)
 #undef WSIM_SELECT
   throw Error(WSIM_ERR_UNSUPPORTED, "no kernel for this pass variant");
@@ -231,11 +262,24 @@ static size_t lds_header(int ncol, int n, int ntw) {
 // contig LDS line pitch: one padding slot every 8 rows, plus one
 static int contig_pitch(int n) { return n + (n >> 3) + 1; }
 
-static int its_for(int C, int R) { return its_per_stage(ept_for(C), C, R); }
+static int its_for(int C, int R, bool compact = false) { return its_per_stage(ept_for(C, compact ? 1 : 0), C, R); }
+
+// every stage of the plan covers ncol lines of length n within the
+// butterflies-per-thread budget of the kernel family
+static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, bool compact, bool contig) {
+  const int NT = threads_for(contig ? 1 : 0);
+  for (int s = 0; s < pl.nst; ++s) {
+    const int R = pl.f[s].R;
+    const long long need = ((long long)ncol * (n / R) + NT - 1) / NT;
+    if (need > its_for(C, R, compact)) return false;
+  }
+  return true;
+}
 
 struct Launch {
   KernelFn fn;
   dim3 grid;
+  int threads;
   size_t lds;
   PassArgs args;
   const PassArgs* dargs;  // device copy of args
@@ -331,11 +375,7 @@ struct Solver {
     a.rstride = contig ? 1 : a.inner;
     a.plan = pl;
     auto fits = [&](int ncol) {
-      for (int s = 0; s < pl.nst; ++s) {
-        const int R = pl.f[s].R;
-        const long long need = ((long long)ncol * (n / R) + NT - 1) / NT;
-        if (need > its_for(C, R)) return false;
-      }
+      if (!tile_fits(pl, n, ncol, C, false, contig)) return false;
       size_t bytes;
       if (contig)
         bytes = (size_t)C * ncol * contig_pitch(n) * sizeof(float2);
@@ -418,8 +458,16 @@ struct Solver {
     L.args.twi = d_twi[axis];
     int blocks = 0;
     configure(L.args, contig, mode, axis, blocks, L.lds);
-    L.fn = select_kernel(contig, mode, pro, epi, C);
+    // the compact register family whenever the tile fits it (WSIM_COMPACT=0 disables)
+    const char* env_k = std::getenv("WSIM_COMPACT");
+    const bool compact = !(env_k && std::atoi(env_k) == 0) && tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, true, contig);
+    L.fn = select_kernel(contig, mode, pro, epi, C, compact);
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
+    L.threads = threads_for(contig ? 1 : 0);
+    L.args.blocks = blocks;
+    L.args.nrhs = nrhs;
+    L.args.grp_n = 1;
+    L.args.grp_end = nullptr;
     L.name = name;
     const double perv = (epi == EPI_MIX) ? (40.0 * C + 8.0) : 16.0 * C;
     L.bytes = perv * (double)nvox * nrhs;
@@ -487,7 +535,7 @@ struct Solver {
   }
 
   void launch(const Launch& L) {
-    hipLaunchKernelGGL(L.fn, L.grid, dim3(NT), L.lds, stream, L.dargs);
+    hipLaunchKernelGGL(L.fn, L.grid, dim3(L.threads), L.lds, stream, L.dargs);
     HIPCHK(hipGetLastError());
   }
 
@@ -567,22 +615,28 @@ struct Solver {
   }
 
   // mean time (ms) of one steady-state iteration (every rhs forced to run)
+  // ms per iteration: the fastest of three timed runs of `reps` iterations
+  // (the minimum filters clock and scheduling noise out of the tuner's choices)
   double time_iteration(int reps) {
-    ensure_history(reps + nw + 2);
+    ensure_history(3 * reps + nw + 2);
     reset_state(1ll << 40, -1.0, true);
     enqueue_iterations(nw);
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, stream));
-    enqueue_iterations(reps);
-    HIPCHK(hipEventRecord(e1, stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return ms / reps;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], stream));
+    for (int k = 1; k < 4; ++k) {
+      enqueue_iterations(reps);
+      HIPCHK(hipEventRecord(ev[k], stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[3]));
+    double best = 1e30;
+    for (int k = 1; k < 4; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
+      best = std::min(best, (double)ms / reps);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return best;
   }
 
   // plan text: "y:8,8,5/16;x:4,16,5/16;z:4,16,5/32" (axis:radices/columns)
@@ -687,17 +741,21 @@ struct Solver {
         } catch (const Error&) {
           t = 1e30;  // the plan does not fit a workgroup
         }
-        if (t < best) {
+        if (t < best * 0.997) {  // a plan change must beat the noise
           best = t;
         } else {
           plan_rad[a] = old_r;
           plan_ncol[a] = old_c;
         }
       };
-      for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a]);
-      const std::vector<int> cand = contig ? std::vector<int>{4, 8, 12, 16, 24, 32, 48}
-                                           : std::vector<int>{8, 16, 32};
-      for (int c : cand) try_cfg(plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a], c);
+      // two rounds: the register family a tile can use depends on both the
+      // radix order and the tile width
+      const std::vector<int> cand = contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32, 48}
+                                           : std::vector<int>{4, 8, 16, 32};
+      for (int round = 0; round < 2; ++round) {
+        for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a]);
+        for (int c : cand) try_cfg(plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a], c);
+      }
       build_tables(a);
     }
     rebuild_launches();
@@ -713,7 +771,7 @@ struct Solver {
   }
 
   void launch_control() {
-    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(256), 0, stream, dst, partials, nblk);
+    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kControlThreads), 0, stream, dst, partials, nblk);
     HIPCHK(hipGetLastError());
   }
 
@@ -1100,6 +1158,189 @@ static int guard(F&& f) {
   }
 }
 
+
+// ----------------------------------------------------------------------------
+// groups: several independent problems (any grid sizes) advanced together.
+// Problems whose iterations launch the same kernel sequence share one
+// launch per pass (up to 64 problems per launch, each owning a range of
+// workgroups); one control launch serves every problem.  A replayed graph
+// of B iterations therefore costs (passes + 1) launches per iteration
+// whatever the number of problems.
+// ----------------------------------------------------------------------------
+constexpr int kGroupMax = 64;
+
+struct Group {
+  std::vector<Solver*> ps;
+  hipStream_t stream = nullptr;
+  struct Sub {
+    std::vector<int> members;
+    std::vector<Launch> launches;
+    PassArgs* d_tab = nullptr;  // [passes][members]
+    int* d_end = nullptr;       // [passes][members]
+  };
+  std::vector<Sub> subs;
+  CtlEntry* d_ctl = nullptr;
+  int* d_done = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  ~Group() { release(); }
+
+  void release_tables() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    for (auto& sb : subs) {
+      if (sb.d_tab) (void)hipFree(sb.d_tab);
+      if (sb.d_end) (void)hipFree(sb.d_end);
+    }
+    subs.clear();
+    if (d_ctl) (void)hipFree(d_ctl);
+    if (d_done) (void)hipFree(d_done);
+    d_ctl = nullptr;
+    d_done = nullptr;
+  }
+
+  void release() {
+    release_tables();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  void create(Solver* const* s, int n) {
+    require(n >= 1, "a group needs at least one solver");
+    for (int i = 0; i < n; ++i) {
+      require(s[i] != nullptr, "null solver in group");
+      for (int j = 0; j < i; ++j) require(s[j] != s[i], "a solver appears twice in the group");
+      ps.push_back(s[i]);
+    }
+    HIPCHK(hipSetDevice(ps[0]->dev));
+    for (Solver* p : ps) require(p->dev == ps[0]->dev, "every solver of a group must be on one device");
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
+
+  // grouped launch tables from the solvers' current launches (sources,
+  // plans and media may have changed since the last solve)
+  void build() {
+    release_tables();
+    std::vector<int> todo(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i) todo[i] = (int)i;
+    auto same_seq = [&](int a, int b) {
+      const auto &A = ps[a]->iter, &B = ps[b]->iter;
+      if (A.size() != B.size()) return false;
+      for (size_t k = 0; k < A.size(); ++k)
+        if (A[k].fn != B[k].fn || A[k].threads != B[k].threads) return false;
+      return true;
+    };
+    while (!todo.empty()) {
+      Sub sb;
+      std::vector<int> rest;
+      for (int i : todo) {
+        if ((sb.members.empty() || same_seq(sb.members[0], i)) && (int)sb.members.size() < kGroupMax)
+          sb.members.push_back(i);
+        else
+          rest.push_back(i);
+      }
+      todo.swap(rest);
+      const size_t K = ps[sb.members[0]]->iter.size(), M = sb.members.size();
+      std::vector<PassArgs> tab(K * M);
+      std::vector<int> ends(K * M);
+      Solver::dalloc(&sb.d_tab, K * M);
+      Solver::dalloc(&sb.d_end, K * M);
+      for (size_t k = 0; k < K; ++k) {
+        Launch G = ps[sb.members[0]]->iter[k];
+        long long x = 0;
+        unsigned y = 1;
+        size_t lds = 0;
+        for (size_t m = 0; m < M; ++m) {
+          const Launch& L = ps[sb.members[m]]->iter[k];
+          x += L.grid.x;
+          require(x < (1ll << 31), "group too large for one launch");
+          y = std::max(y, L.grid.y);
+          lds = std::max(lds, L.lds);
+          ends[k * M + m] = (int)x;
+          PassArgs a = L.args;
+          a.grp_n = (int)M;
+          a.grp_end = sb.d_end + k * M;
+          tab[k * M + m] = a;
+        }
+        G.grid = dim3((unsigned)x, y, 1);
+        G.lds = lds;
+        G.dargs = sb.d_tab + k * M;
+        if (G.lds > 48 * 1024)
+          HIPCHK(hipFuncSetAttribute((const void*)G.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G.lds));
+        sb.launches.push_back(G);
+      }
+      HIPCHK(hipMemcpy(sb.d_tab, tab.data(), tab.size() * sizeof(PassArgs), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(sb.d_end, ends.data(), ends.size() * sizeof(int), hipMemcpyHostToDevice));
+      subs.push_back(std::move(sb));
+    }
+    Solver::dalloc(&d_done, ps.size());
+    Solver::dalloc(&d_ctl, ps.size());
+    std::vector<CtlEntry> ce(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i)
+      ce[i] = CtlEntry{ps[i]->dst, ps[i]->partials, ps[i]->nblk, d_done + i};
+    HIPCHK(hipMemcpy(d_ctl, ce.data(), ce.size() * sizeof(CtlEntry), hipMemcpyHostToDevice));
+  }
+
+  void enqueue_iterations(int k) {
+    for (int i = 0; i < k; ++i) {
+      for (const auto& sb : subs)
+        for (const auto& L : sb.launches) {
+          hipLaunchKernelGGL(L.fn, L.grid, dim3(L.threads), L.lds, stream, L.dargs);
+          HIPCHK(hipGetLastError());
+        }
+      hipLaunchKernelGGL(control_group_kernel, dim3((unsigned)ps.size()), dim3(kControlThreads), 0, stream,
+                         (const CtlEntry*)d_ctl);
+      HIPCHK(hipGetLastError());
+    }
+  }
+
+  // every problem from E' = dE' = 0 (or its warm start) to its own stop
+  void solve(const int64_t* max_iter, const double* thr, long long batch) {
+    int nw = 1;
+    double it_us = 0.0;
+    for (size_t i = 0; i < ps.size(); ++i) {
+      Solver* p = ps[i];
+      require(max_iter[i] >= 1, "max_iter must be >= 1");
+      if (p->iter.empty() || !p->d_args) p->rebuild_launches();
+      p->ensure_history((int)std::min<long long>(max_iter[i], 2000000) + 1);
+      p->reset_state(max_iter[i], thr[i]);
+      nw = std::max(nw, p->nw);
+      it_us += 112.0 * (double)p->nvox * p->C * p->nrhs / 5e6;
+    }
+    build();
+    if (batch <= 0) {
+      long long k = (long long)std::ceil(2000.0 / std::max(it_us, 1.0));
+      k = std::max<long long>(nw, std::min<long long>(k, 256));
+      batch = ((k + nw - 1) / nw) * nw;
+    }
+    std::vector<int> done(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i) {
+      HIPCHK(hipMemcpy(&ps[i]->host_state, ps[i]->dst, sizeof(DevState), hipMemcpyDeviceToHost));
+      done[i] = ps[i]->host_state.all_stopped;
+    }
+    HIPCHK(hipMemcpy(d_done, done.data(), done.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    enqueue_iterations((int)batch);
+    HIPCHK(hipStreamEndCapture(stream, &graph));
+    HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    for (;;) {
+      bool all = true;
+      for (int d : done) all = all && d;
+      if (all) break;
+      HIPCHK(hipGraphLaunch(gexec, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipMemcpy(done.data(), d_done, done.size() * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (Solver* p : ps) {
+      HIPCHK(hipMemcpy(&p->host_state, p->dst, sizeof(DevState), hipMemcpyDeviceToHost));
+      p->last_iters = p->host_state.it - 1;
+    }
+  }
+};
+
 }  // namespace wsim
 
 using namespace wsim;
@@ -1287,6 +1528,47 @@ int wsim_solver_set_reference_energy(wsim_solver* s, int rhs, double energy) {
   return guard([&]() -> int {
     require(s != nullptr, "null solver");
     s->s.set_reference_energy(rhs, energy);
+    return WSIM_OK;
+  });
+}
+
+struct wsim_group {
+  Group g;
+};
+
+int wsim_group_create(wsim_group** out, wsim_solver* const* solvers, int n) {
+  return guard([&]() -> int {
+    require(out != nullptr && solvers != nullptr, "null argument");
+    require(n >= 1 && n <= 4096, "group size out of range");
+    std::vector<Solver*> v(n);
+    for (int i = 0; i < n; ++i) {
+      require(solvers[i] != nullptr, "null solver in group");
+      v[i] = &solvers[i]->s;
+    }
+    auto* g = new wsim_group();
+    try {
+      g->g.create(v.data(), n);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+    return WSIM_OK;
+  });
+}
+
+int wsim_group_solve(wsim_group* g, const int64_t* max_iter, const double* energy_threshold, int64_t batch) {
+  return guard([&]() -> int {
+    require(g != nullptr && max_iter != nullptr && energy_threshold != nullptr, "null argument");
+    HIPCHK(hipSetDevice(g->g.ps[0]->dev));
+    g->g.solve(max_iter, energy_threshold, batch);
+    return WSIM_OK;
+  });
+}
+
+int wsim_group_destroy(wsim_group* g) {
+  return guard([&]() -> int {
+    delete g;
     return WSIM_OK;
   });
 }

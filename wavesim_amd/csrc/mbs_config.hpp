@@ -4,10 +4,18 @@
 
 namespace wsim {
 
-constexpr int kThreads = 256;  // threads per workgroup of every axis pass
-// registers (complex values) per thread: 32 for scalar fields, 24 for the
-// 3-component vector field
-constexpr int ept_for(int C) { return C == 1 ? 32 : 24; }
+#ifndef WSIM_STRIDED_THREADS
+#define WSIM_STRIDED_THREADS 512
+#endif
+// threads per workgroup: 256 for the contiguous-axis passes; the strided
+// passes hold wide tiles (LDS-limited to a few workgroups per CU), so they
+// use larger workgroups to keep more waves resident
+constexpr int threads_for(int contig) { return contig ? 256 : WSIM_STRIDED_THREADS; }
+// registers (complex values) per thread and stage: 32 for scalar fields, 24
+// for the 3-component vector field; the compact family halves them, which
+// lowers the kernel's register count (set by its largest radix path) and
+// raises occupancy for tiles that fit
+constexpr int ept_for(int C, int compact) { return C == 1 ? (compact ? 16 : 32) : (compact ? 12 : 24); }
 
 using KernelFn = void (*)(const PassArgs*);
 
@@ -24,9 +32,11 @@ using KernelFn = void (*)(const PassArgs*);
   X(0, 1, 0, 0)                  \
   X(0, 2, 0, 0)
 
-#define WSIM_DECLARE_KPTR(ct, m, p, e)              \
-  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_1(); \
-  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_3();
+#define WSIM_DECLARE_KPTR(ct, m, p, e)                \
+  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_1_0(); \
+  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_1_1(); \
+  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_3_0(); \
+  KernelFn wsim_kptr_##ct##_##m##_##p##_##e##_3_1();
 WSIM_FOR_EACH_VARIANT(WSIM_DECLARE_KPTR)
 
 }  // namespace wsim

@@ -112,7 +112,7 @@ struct PassArgs {
   float2* dE;
   float2* E;
   const float2* gamma;
-  double* partials;  // [rhs][gridDim.x]
+  double* partials;  // [rhs][blocks]
   const DevState* st;
   const float* pe[3];     // [nw][N[a]] scaled Fourier coordinates
   const float2* ramp[3];  // [nw][N[a]] wiggle phase ramps
@@ -121,6 +121,13 @@ struct PassArgs {
   float invN;
   int roi_lo[4];  // 0-based inclusive, [y x z pol]
   int roi_hi[4];
+  // one launch may carry several problems (entries of an array of PassArgs):
+  // entry e owns workgroups [grp_end[e-1], grp_end[e]) along x.  Read from
+  // entry 0 only.
+  int grp_n;           // entries in the launch (1: a single problem), <= 64
+  const int* grp_end;  // [grp_n] exclusive prefix sums of the entries' widths
+  int blocks;          // this entry's workgroups along x (row length of partials)
+  int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
 };
 
 // butterflies per thread per stage: EPT registers' worth, at most kMaxIts
@@ -147,6 +154,10 @@ __device__ __forceinline__ float2 ldg(const float2* base, uint32_t i) {
 __device__ __forceinline__ void stg(float2* base, uint32_t i, float2 v) {
   *reinterpret_cast<float2*>(reinterpret_cast<char*>(base) + (size_t)(i * 8u)) = v;
 }
+
+#ifndef WSIM_EPI_CHUNK  // rows of one column whose mix operands are loaded together
+#define WSIM_EPI_CHUNK 4
+#endif
 
 template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
 struct Pass {
@@ -192,13 +203,14 @@ struct Pass {
   __device__ __forceinline__ const ColInfo& col(int t) const { return CONTIG ? cols[t] : cfix; }
 
   // ---- element transforms --------------------------------------------------
-  __device__ __forceinline__ float2 load_first(const ColInfo& ci, int c, int row) const {
-    const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+  // first-stage operand before the ramps: dE' (+ W0/Nw in the warm steps,
+  // + i/eps/Nw * S in the source steps)
+  __device__ __forceinline__ float2 load_raw(uint32_t off, int c, int row) const {
+    const uint32_t i = off + (uint32_t)row * (uint32_t)a.rstride;
     float2 v = ldg(in_b[c], i);
     if (PRO == PRO_SRC_RAMP) {
       if (warm_step) v = cadd(v, cscale(ldg(W0_b[c], i), inv_nw));
       if (add_src) v = cadd(v, ldg(S_b[c], i));
-      v = cmul(v, cmul(ramp_a[row], ci.rc));
     }
     return v;
   }
@@ -216,6 +228,8 @@ struct Pass {
   static constexpr int its() { return its_per_stage(EPT, C, R); }
 
   // ---- stage pieces (C components, radix R) --------------------------------
+  // first stage from HBM.  Only loads here: a use of a loaded value inside
+  // the guarded block would make every iteration wait for its own loads.
   template <int R>
   __device__ __forceinline__ void stage_load_global(float2* v, const Stage& s) const {
     constexpr int ITS = its<R>();
@@ -232,7 +246,24 @@ struct Pass {
       for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          v[(c * ITS + it) * R + r] = ok ? load_first(ci, c, j + r * s.nb) : make_float2(0.f, 0.f);
+          v[(c * ITS + it) * R + r] = ok ? load_raw(ci.off, c, j + r * s.nb) : make_float2(0.f, 0.f);
+    }
+    if (PRO == PRO_SRC_RAMP) {
+      // wiggle ramps of the first forward pass (WaveSim.m:38-47)
+#pragma unroll
+      for (int it = 0; it < ITS; ++it) {
+        const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+        if (b < total) {
+          int t, j;
+          decode(b, s, t, j);
+          const float2 rc = col(t).rc;
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              v[(c * ITS + it) * R + r] = cmul(v[(c * ITS + it) * R + r], cmul(ramp_a[j + r * s.nb], rc));
+        }
+      }
     }
   }
 
@@ -296,7 +327,6 @@ struct Pass {
   __device__ __forceinline__ void stage_store_global(const float2* v, const Stage& s) {
     constexpr int ITS = its<R>();
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
-    const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
       const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
@@ -305,6 +335,10 @@ struct Pass {
         decode(b, s, t, j);
         const ColInfo& ci = col(t);
         if (!ci.valid) continue;
+        if (EPI == EPI_MIX) {
+          mix_rows<R>(v, it, j, s.nb, ci);
+          continue;
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
@@ -312,38 +346,63 @@ struct Pass {
           if (EPI == EPI_NONE) {
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
-          } else {
+          } else {  // EPI_RAMP: undo the ramps only (propagation probe)
             const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
-            if (EPI == EPI_RAMP) {
 #pragma unroll
-              for (int c = 0; c < C; ++c) stg(out_b[c], i, cmul(v[(c * ITS + it) * R + r], ramp));
-            } else {
-              // f_mix.m:3-4 with alpha = 0.7 : (1 - a g) dE - (0.7 i g^2) Eprop
-              const float2 gm = ldg(gam_b, i);
-              const float2 c1 = make_float2(1.f - 0.7f * gm.x, -0.7f * gm.y);
-              const float2 g2 = cmul(gm, gm);
-              const float2 c2 = make_float2(-0.7f * g2.y, 0.7f * g2.x);
-              const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
-#pragma unroll
-              for (int c = 0; c < C; ++c) {
-                const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
-                float2 d0 = ldg(dE_b[c], i);
-                const float2 e0 = ldg(E_b[c], i);
-                // warm start (first Nw steps): the increment of the correction
-                // W0/Nw, i.e. (M_w - 1) W0/Nw, joins the recurrence
-                float2 w = make_float2(0.f, 0.f);
-                if (warm_step) {
-                  w = cscale(ldg(W0_b[c], i), inv_nw);
-                  d0 = cadd(d0, w);
-                }
-                const float2 dn = csub(csub(cmul(c1, d0), cmul(c2, ep)), w);
-                stg(dE_b[c], i, dn);
-                stg(E_b[c], i, cadd(e0, dn));
-                if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
-                  eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
-              }
-            }
+            for (int c = 0; c < C; ++c) stg(out_b[c], i, cmul(v[(c * ITS + it) * R + r], ramp));
           }
+        }
+      }
+    }
+  }
+
+  // mix + accumulate of the rows j + r*nb of one column (f_mix.m:3-4 with
+  // alpha = 0.7: dE' <- (1 - a g) dE' - (0.7 i g^2) Eprop; E' <- E' + dE').
+  // The operands of a chunk of rows are loaded before any of its stores:
+  // a load after a store to a possibly aliasing address would wait for the
+  // previous row's whole load-compute-store chain.
+  template <int R>
+  __device__ __forceinline__ void mix_rows(const float2* v, int it, int j, int nb, const ColInfo& ci) {
+    constexpr int ITS = its<R>();
+    constexpr int CH = WSIM_EPI_CHUNK < R ? WSIM_EPI_CHUNK : R;
+    const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
+#pragma unroll
+    for (int r0 = 0; r0 < R; r0 += CH) {
+      float2 gm[CH], d0[CH][C], e0[CH][C], w0[CH][C];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        if (r0 + q >= R) break;
+        const uint32_t i = ci.off + (uint32_t)(j + (r0 + q) * nb) * (uint32_t)a.rstride;
+        gm[q] = ldg(gam_b, i);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          d0[q][c] = ldg(dE_b[c], i);
+          e0[q][c] = ldg(E_b[c], i);
+          w0[q][c] = warm_step ? cscale(ldg(W0_b[c], i), inv_nw) : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        if (r0 + q >= R) break;
+        const int r = r0 + q;
+        const int row = j + r * nb;
+        const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+        const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
+        const float2 c1 = make_float2(1.f - 0.7f * gm[q].x, -0.7f * gm[q].y);
+        const float2 g2 = cmul(gm[q], gm[q]);
+        const float2 c2 = make_float2(-0.7f * g2.y, 0.7f * g2.x);
+        const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
+          // warm start (first Nw steps): the increment of the correction
+          // W0/Nw, i.e. (M_w - 1) W0/Nw, joins the recurrence
+          const float2 w = w0[q][c];
+          const float2 dn = csub(csub(cmul(c1, cadd(d0[q][c], w)), cmul(c2, ep)), w);
+          stg(dE_b[c], i, dn);
+          stg(E_b[c], i, cadd(e0[q][c], dn));
+          if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
+            eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
         }
       }
     }
@@ -414,9 +473,20 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   // the arguments live in device memory (read through the scalar cache):
   // indexing the stage table of a by-value kernel argument would copy the
   // whole struct to scratch
+  int bx = (int)blockIdx.x;
+  if (ap->grp_n > 1) {
+    // grouped launch: one lane per entry compares the block index with the
+    // entry's end; the count of passed ends is this workgroup's entry
+    const int lane = (int)(threadIdx.x & 63);
+    const int end = lane < ap->grp_n ? ap->grp_end[lane] : 0x7fffffff;
+    const int e = (int)__popcll(__ballot(bx >= end));
+    if (e > 0) bx -= ap->grp_end[e - 1];
+    ap += e;
+  }
   const PassArgs& a = *ap;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rhs = blockIdx.y;
+  if (rhs >= a.nrhs) return;
   const DevState* st = a.st;
   if (st->stopped[rhs]) return;
 
@@ -445,10 +515,10 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   long long vblk;
   int o = 0, itile = 0;
   if (CONTIG) {
-    vblk = (long long)blockIdx.x * a.ncol * a.n;
+    vblk = (long long)bx * a.ncol * a.n;
   } else {
-    o = (int)fdiv(blockIdx.x, a.ftile);
-    itile = (int)blockIdx.x - o * a.ntile_inner;
+    o = (int)fdiv((uint32_t)bx, a.ftile);
+    itile = bx - o * a.ntile_inner;
     vblk = (long long)o * a.n * a.inner + (long long)itile * a.ncol;
   }
 
@@ -462,7 +532,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     int crd[3] = {0, 0, 0};
     int valid;
     if (CONTIG) {
-      const int line = (int)blockIdx.x * a.ncol + t;  // < nvox < 2^31
+      const int line = bx * a.ncol + t;  // < nvox < 2^31
       valid = line < a.nlines;
       ci.off = (uint32_t)(t * a.n);
       int rem = valid ? line : 0;
@@ -623,7 +693,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int i = 0; i < NT / 64; ++i) sum += red[i];
-      a.partials[(size_t)rhs * gridDim.x + blockIdx.x] = sum;
+      a.partials[(size_t)rhs * a.blocks + bx] = sum;
     }
   }
 }

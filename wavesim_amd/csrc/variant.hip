@@ -1,16 +1,16 @@
 // One instantiation of axis_pass_kernel, selected by -D flags at build time
-// (WSIM_CONTIG, WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C), so that the variants
-// compile in parallel.  The host runtime gets the kernel through
-// wsim_kptr_<contig>_<mode>_<pro>_<epi>_<C>().
+// (WSIM_CONTIG, WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, WSIM_COMPACT), so
+// that the variants compile in parallel.  The host runtime gets the kernel
+// through wsim_kptr_<contig>_<mode>_<pro>_<epi>_<C>_<compact>().
 #include "mbs_kernels.hpp"
 #include "mbs_config.hpp"
 
-#define WSIM_CAT6(a, b, c, d, e, f) a##_##b##_##c##_##d##_##e##_##f
-#define WSIM_NAME(a, b, c, d, e, f) WSIM_CAT6(a, b, c, d, e, f)
+#define WSIM_CAT7(a, b, c, d, e, f, g) a##_##b##_##c##_##d##_##e##_##f##_##g
+#define WSIM_NAME(a, b, c, d, e, f, g) WSIM_CAT7(a, b, c, d, e, f, g)
 
 namespace wsim {
-KernelFn WSIM_NAME(wsim_kptr, WSIM_CONTIG, WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C)() {
-  return axis_pass_kernel<(WSIM_CONTIG != 0), WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, kThreads,
-                          ept_for(WSIM_C)>;
+KernelFn WSIM_NAME(wsim_kptr, WSIM_CONTIG, WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, WSIM_COMPACT)() {
+  return axis_pass_kernel<(WSIM_CONTIG != 0), WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, threads_for(WSIM_CONTIG),
+                          ept_for(WSIM_C, WSIM_COMPACT)>;
 }
 }  // namespace wsim

@@ -363,6 +363,12 @@ class WaveSimBase:
             cb = lambda it, hist: self.callback(it, hist)
         sv.solve(mi, self.energy_threshold, callback=cb,
                  callback_interval=float(self.callback_interval))
+        return self._finish(sv, prepared, t0)
+
+    def propagate(self, E, wiggle: int = 0):
+        return self.solver().propagate(wiggle, E)
+
+    def _finish(self, sv, prepared, t0):
         Es, states = [], []
         g4 = self.gamma[..., None]
         for r, src in enumerate(prepared):
@@ -378,8 +384,34 @@ class WaveSimBase:
             states.append(st)
         return Es, states
 
-    def propagate(self, E, wiggle: int = 0):
-        return self.solver().propagate(wiggle, E)
+
+def exec_many(problems, max_iterations: Optional[int] = None):
+    """Solve several independent problems -- (simulation, source) pairs with
+    any grids and media -- in one schedule (a group of solvers sharing
+    launches).  Returns [(E, state), ...]; each equals sim.exec(source).
+    The reference solves such sets as a loop of exec calls
+    (examples/pulse2d.m:31-40, examples/phase_conjugation.m:48-56)."""
+    from .solver import SolverGroup
+    t0 = time.perf_counter()
+    sims = [p[0] for p in problems]
+    if len(set(id(s) for s in sims)) != len(sims):
+        raise ValueError("exec_many: each simulation may appear once (use exec_batch for several "
+                         "sources on one medium)")
+    prepared, svs = [], []
+    for sim, src in problems:
+        ps = sim._prepare(src)
+        sv = sim.solver(1)
+        sv.set_source(0, ps.positions, ps.values)
+        prepared.append(ps)
+        svs.append(sv)
+    mi = [int(max_iterations) if max_iterations is not None else s.max_iterations() for s in sims]
+    with SolverGroup(svs) as g:
+        g.solve(mi, [s.energy_threshold for s in sims])
+    out = []
+    for sim, sv, ps in zip(sims, svs, prepared):
+        Es, states = sim._finish(sv, [ps], t0)
+        out.append((Es[0], states[0]))
+    return out
 
 
 class WaveSim(WaveSimBase):

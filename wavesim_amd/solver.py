@@ -178,3 +178,46 @@ class MBSSolver:
         nk = C.c_int()
         capi.check(self._lib.wsim_solver_profile(self._h, int(reps), K, ms, by, nm, C.byref(nk)))
         return [(nm[i].decode(), ms[i], by[i]) for i in range(min(nk.value, K))]
+
+
+class SolverGroup:
+    """Independent problems (MBSSolver objects of any grid sizes) advanced
+    together: their passes share launches and one control launch serves all
+    of them (wsim_group_* in include/wavesim_mbs.h).  After ``solve`` every
+    solver's ``result``/``field`` read exactly as after its own ``solve``.
+    The reference runs several problems as a loop of exec calls
+    (examples/pulse2d.m:31-40)."""
+
+    def __init__(self, solvers: Sequence[MBSSolver]):
+        L = capi.lib()
+        self.solvers = list(solvers)
+        hs = (C.c_void_p * len(self.solvers))(*[s._h for s in self.solvers])
+        h = C.c_void_p()
+        capi.check(L.wsim_group_create(C.byref(h), hs, len(self.solvers)))
+        self._h = h
+        self._lib = L
+
+    def solve(self, max_iter, energy_threshold, batch: int = 0):
+        """max_iter, energy_threshold: one value for all problems or one per problem."""
+        n = len(self.solvers)
+        mi = np.broadcast_to(np.asarray(max_iter, dtype=np.int64), (n,)).copy()
+        th = np.broadcast_to(np.asarray(energy_threshold, dtype=np.float64), (n,)).copy()
+        capi.check(self._lib.wsim_group_solve(self._h, mi.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              th.ctypes.data_as(C.POINTER(C.c_double)), int(batch)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wsim_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
