@@ -527,6 +527,14 @@ struct Solver {
       LE.args.W0 = probe_mode ? nullptr : W0;
       out.push_back(LE);
     }
+    // tile order: consecutive passes walk memory in opposite directions, so
+    // each pass starts on what its predecessor wrote last (still in the
+    // memory-side cache).  Even passes but the last run backwards; with an
+    // odd pass count only the pair (last, next iteration's first) gets none.
+    // WSIM_REVERSE=<mask> overrides (bit k: pass k backwards).
+    const char* rv = std::getenv("WSIM_REVERSE");
+    for (size_t k = 0; k < out.size(); ++k)
+      out[k].args.reverse = rv ? (std::atoi(rv) >> k) & 1 : (k % 2 == 0 && k + 1 < out.size()) ? 1 : 0;
     if (probe_mode)
       for (auto& L : out) {
         L.args.st = dprobe;

@@ -128,6 +128,8 @@ struct PassArgs {
   const int* grp_end;  // [grp_n] exclusive prefix sums of the entries' widths
   int blocks;          // this entry's workgroups along x (row length of partials)
   int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
+  int reverse;         // walk the tiles from the end of memory (the part the
+                       // previous pass wrote last, still in the memory-side cache)
 };
 
 // butterflies per thread per stage: EPT registers' worth, at most kMaxIts
@@ -484,6 +486,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     ap += e;
   }
   const PassArgs& a = *ap;
+  if (a.reverse) bx = a.blocks - 1 - bx;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rhs = blockIdx.y;
   if (rhs >= a.nrhs) return;
