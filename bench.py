@@ -109,6 +109,29 @@ def load_traffic(workload: str, plan: str):
     return out if file_plan == plan else {}
 
 
+def learned_ab(media: int):
+    """A/B of the learned warm start on held-out sphere media (128^3 ROI, 192^3
+    grid, plane wave; seeds 0.. are not in the training set 1000-1599): zero
+    start vs straight-ray prior vs learned warm start, each to the zero start's
+    absolute residual, timed from inputs on the GPU including the prior and
+    network inference."""
+    from wavesim_amd.learned.warmstart import DEFAULT_WEIGHTS, WarmStart, evaluate
+    ws = WarmStart.load(DEFAULT_WEIGHTS)
+    rows, _ = evaluate(ws, range(media))
+    zs = sum(r["zero_s"] for r in rows)
+    out = {"workload": "spheres_3d_128: 1-8 random spheres (n 1.2-1.6 in 1.33), plane wave, "
+                       "grid 192^3, residual 1e-3", "media": len(rows),
+           "weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT), "parameters": ws.net.nparam}
+    for k in ("zero", "prior", "learned"):
+        out[f"{k}_iterations"] = float(np.mean([r[f"{k}_its"] for r in rows]))
+        out[f"{k}_ms"] = round(1e3 * float(np.mean([r[f"{k}_s"] for r in rows])), 3)
+        out[f"{k}_rel_err_vs_tight"] = float(np.mean([r[f"{k}_err"] for r in rows]))
+    out["learned_inference_ms"] = round(1e3 * float(np.mean([r["learned_predict_s"] for r in rows])), 3)
+    out["speedup_learned"] = round(zs / sum(r["learned_s"] for r in rows), 4)
+    out["speedup_prior"] = round(zs / sum(r["prior_s"] for r in rows), 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +143,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="keep the default FFT plan when none is cached")
     ap.add_argument("--plan", default="", help="FFT plan text to use instead of the cached/tuned one")
+    ap.add_argument("--learned-media", type=int, default=4,
+                    help="held-out media of the learned warm-start A/B (0: skip)")
     ap.add_argument("--trace-iterations", type=int, default=0,
                     help="run this many iterations of the workload and exit (counter collection)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -193,6 +218,11 @@ def main():
 
     traffic = load_traffic(args.workload, sv.plan).get(dom[0])
 
+    learned = None
+    if args.learned_media > 0 and not vector:
+        learned = learned_ab(args.learned_media)
+        print(f"# learned warm start: {learned}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nmap, opt, source, its, args.cpu_seconds, vector=vector)
@@ -213,7 +243,7 @@ def main():
         "config": {"workload": f"{args.workload}: grid {sim.N[0]}x{sim.N[1]}x{sim.N[2]}x{C} "
                                f"(interior + ARL boundary), Nw={len(sim.wiggles)}, plain MBS from "
                                f"E=0 to residual 1e-3 (energy ratio {sim.energy_threshold:g})",
-                   "iterations": its, "grid": sim.N, "learned_warm_start": None,
+                   "iterations": its, "grid": sim.N, "learned_warm_start": learned,
                    "fft_plan": sv.plan, "fft_plan_source": tuned},
         "iterations_per_s": round(it_per_s, 2),
         "voxel_updates_per_s": round(C * nvox * it_per_s, 1),

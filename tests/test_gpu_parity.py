@@ -183,6 +183,20 @@ def test_odd_radices(gpu, shape):
     assert rel_l2(g.propagate(E, 0), mbs.propagate(o, E, o.wiggles[0])) <= TOL
 
 
+@pytest.mark.parametrize("shape", [(16, 2112), (2112, 8), (24, 3072)])
+def test_long_axes(gpu, shape):
+    """Axes whose one-column tile with its tables exceeds the usual per-workgroup
+    LDS target (2112 = 2^6 3 11 strided and contiguous; 3072 strided)."""
+    from oracle import mbs
+    nmap = cases.smooth_disorder(shape, 4, absorption=0.05)
+    opt = cases.base_opt([0, 0], 1e-8)
+    o, g = _pair(nmap, opt)
+    rng = np.random.default_rng(3)
+    E = (rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)).astype(np.complex64)
+    for w in range(len(o.wiggles)):
+        assert rel_l2(g.propagate(E, w), mbs.propagate(o, E, o.wiggles[w])) <= TOL
+
+
 def test_custom_roi(gpu):
     n, opt, src = cases.case_2d_disordered()
     opt = dict(opt, roi=[[5, 7], [30, 44]])

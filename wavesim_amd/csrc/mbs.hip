@@ -326,7 +326,8 @@ struct Solver {
   PassArgs* d_args = nullptr;  // device copies of iter[k].args
   std::vector<Launch> probe_iter;
   PassArgs* d_probe_args = nullptr;
-  size_t lds_limit = 64 * 1024;
+  size_t lds_limit = 64 * 1024;    // per-workgroup LDS the tile search aims at
+  size_t lds_device = 64 * 1024;   // the device's per-workgroup maximum (long axes)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
@@ -374,6 +375,7 @@ struct Solver {
     for (int b = 0; b < axis; ++b) a.inner *= N[b];
     a.rstride = contig ? 1 : a.inner;
     a.plan = pl;
+    size_t lds_cap = lds_limit;
     auto fits = [&](int ncol) {
       if (!tile_fits(pl, n, ncol, C, false, contig)) return false;
       size_t bytes;
@@ -382,8 +384,11 @@ struct Solver {
       else
         bytes = (size_t)n * (C * ncol + 1) * sizeof(float2);
       bytes += lds_header(ncol, n, ntw);
-      return bytes <= lds_limit;
+      return bytes <= lds_cap;
     };
+    // long axes whose single-column tile exceeds the target take up to the
+    // device's per-workgroup LDS (one workgroup per CU)
+    if (!fits(1)) lds_cap = lds_device;
     int ncol = 0;
     const char* env_c = std::getenv(contig ? "WSIM_NCOL_CONTIG" : "WSIM_NCOL_STRIDED");
     int cap = env_c ? std::max(1, std::atoi(env_c)) : (contig ? 16 : 32);
@@ -827,6 +832,7 @@ struct Solver {
       HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
       const char* env_l = std::getenv("WSIM_LDS_CAP_KB");
       lds_limit = std::min<size_t>((size_t)lim, env_l ? (size_t)std::atoi(env_l) * 1024 : LDS_CAP);
+      lds_device = std::max<size_t>((size_t)lim, lds_limit);
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // tables

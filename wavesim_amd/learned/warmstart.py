@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -31,6 +32,8 @@ from .fno import FNO, Config
 from .prior import green_1d
 
 CIN = 6
+# weights trained on 600 solver-generated sphere media (128^3 ROI, seeds 1000-1599)
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "spheres_128.npz")
 
 
 class WarmStart:

@@ -75,6 +75,26 @@ def spheres_3d(n=128, boundary=32, seed=0, residual=1e-3, n_bg=1.33):
     return nmap, opt, Source(np.ones((n, n, 1)), [1, 1, 1])
 
 
+RAGGED_SIZES = (256, 384, 512, 776, 1036, 2048)  # grids 320..2112 with a 32 px boundary
+
+
+def ragged_2d(count=32, boundary=32, residual=1e-3, sizes=RAGGED_SIZES):
+    """``count`` independent 2-D problems of ragged sizes (cycling through
+    ``sizes``), each its own seeded disordered medium (n = 1 + 0.5 u + 0.01i)
+    with a point source at a seeded position."""
+    out = []
+    for i in range(count):
+        n = sizes[i % len(sizes)]
+        rng = np.random.default_rng(1000 + i)
+        u = ndi.gaussian_filter(rng.random((n, n)), 2.0, mode="wrap")
+        nmap = 1.0 + 0.5 * u + 0.01j
+        opt = dict(pixel_size=0.25, boundary_widths=[boundary, boundary], energy_threshold=residual ** 2,
+                   callback_interval=1e9)
+        pos = [int(x) for x in rng.integers(n // 4, 3 * n // 4, size=2)]
+        out.append((nmap, opt, Source(1.0, pos)))
+    return out
+
+
 WORKLOADS = {
     "disordered_3d_256": lambda: disordered_3d(256, 32),
     "disordered_2d_1024": lambda: disordered_2d(1024, 64),
