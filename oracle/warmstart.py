@@ -3,6 +3,11 @@
 No reference counterpart: these check the project's wfno_wkb_prior,
 wfno_features and wfno_compose_prior kernels.  Layout [z][y][x] (x fastest)
 for ROI arrays, [y][x][z] (MATLAB index order) for the solver grid.
+
+Pinned by closed forms (tests/test_oracle_warmstart.py): a homogeneous medium
+gives the background itself, constant contrast a linear phase, a slab the
+path integral per ray; block means by explicit slices; zero envelope gives the
+prior.
 """
 from __future__ import annotations
 
