@@ -266,8 +266,7 @@ static int its_for(int C, int R, bool compact = false) { return its_per_stage(ep
 
 // every stage of the plan covers ncol lines of length n within the
 // butterflies-per-thread budget of the kernel family
-static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, bool compact, bool contig) {
-  const int NT = threads_for(contig ? 1 : 0);
+static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, bool compact, int NT) {
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.f[s].R;
     const long long need = ((long long)ncol * (n / R) + NT - 1) / NT;
@@ -322,6 +321,7 @@ struct Solver {
   float2 *d_twf[3] = {nullptr, nullptr, nullptr}, *d_twi[3] = {nullptr, nullptr, nullptr};
   std::vector<int> plan_rad[3];    // per-axis radix plan (empty: default)
   int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
+  int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, 256 or 512 (0: default)
   std::vector<Launch> iter;  // one MBS iteration
   PassArgs* d_args = nullptr;  // device copies of iter[k].args
   std::vector<Launch> probe_iter;
@@ -363,8 +363,13 @@ struct Solver {
   }
 
   // pass configuration: columns per workgroup and LDS size
+  int threads_of(int axis, bool contig) const {
+    return plan_threads[axis] > 0 ? plan_threads[axis] : default_threads(contig ? 1 : 0);
+  }
+
   void configure(PassArgs& a, bool contig, int mode, int axis, int& blocks, size_t& lds) {
     const AxisPlan& pl = tab[axis].plan;
+    const int nthr = threads_of(axis, contig);
     const int n = N[axis];
     a.ntwf = (int)tab[axis].twf.size();
     a.ntwi = (int)tab[axis].twi.size();
@@ -377,7 +382,7 @@ struct Solver {
     a.plan = pl;
     size_t lds_cap = lds_limit;
     auto fits = [&](int ncol) {
-      if (!tile_fits(pl, n, ncol, C, false, contig)) return false;
+      if (!tile_fits(pl, n, ncol, C, false, nthr)) return false;
       size_t bytes;
       if (contig)
         bytes = (size_t)C * ncol * contig_pitch(n) * sizeof(float2);
@@ -465,10 +470,10 @@ struct Solver {
     configure(L.args, contig, mode, axis, blocks, L.lds);
     // the compact register family whenever the tile fits it (WSIM_COMPACT=0 disables)
     const char* env_k = std::getenv("WSIM_COMPACT");
-    const bool compact = !(env_k && std::atoi(env_k) == 0) && tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, true, contig);
+    const bool compact = !(env_k && std::atoi(env_k) == 0) && tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, true, threads_of(axis, contig));
     L.fn = select_kernel(contig, mode, pro, epi, C, compact);
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
-    L.threads = threads_for(contig ? 1 : 0);
+    L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
     L.args.nrhs = nrhs;
     L.args.grp_n = 1;
@@ -652,7 +657,8 @@ struct Solver {
     return best;
   }
 
-  // plan text: "y:8,8,5/16;x:4,16,5/16;z:4,16,5/32" (axis:radices/columns)
+  // plan text: "y:8,8,5/16/512;x:4,16,5/16;z:4,16,5/32" (axis:radices/columns[/threads];
+  // columns 0 = widest tile that fits, threads omitted = the default for the axis)
   std::string plan_string() const {
     static const char* nm = "yxz";
     std::string out;
@@ -664,6 +670,7 @@ struct Solver {
       const AxisPlan& p = tab[a].plan;
       for (int s2 = 0; s2 < p.nst; ++s2) out += (s2 ? "," : "") + std::to_string(p.f[s2].R);
       out += "/" + std::to_string(plan_ncol[a]);
+      if (plan_threads[a] > 0) out += "/" + std::to_string(plan_threads[a]);
     }
     return out;
   }
@@ -694,6 +701,10 @@ struct Solver {
       require(prod == N[a], "plan radices do not multiply to the axis length in '" + item + "'");
       plan_rad[a] = rad;
       plan_ncol[a] = slash == std::string::npos ? 0 : std::max(0, std::atoi(item.c_str() + slash + 1));
+      const size_t slash2 = slash == std::string::npos ? std::string::npos : item.find('/', slash + 1);
+      const int thr = slash2 == std::string::npos ? 0 : std::atoi(item.c_str() + slash2 + 1);
+      require(thr == 0 || thr == 256 || thr == 512, "plan workgroup size must be 256 or 512 in '" + item + "'");
+      plan_threads[a] = thr;
       build_tables(a);
     }
     rebuild_launches();
@@ -741,11 +752,12 @@ struct Solver {
     for (int a : axes) {
       if (N[a] == 1) continue;
       const bool contig = (a == axes[0]);
-      auto try_cfg = [&](const std::vector<int>& rad, int ncol) {
+      auto try_cfg = [&](const std::vector<int>& rad, int ncol, int thr) {
         const std::vector<int> old_r = plan_rad[a];
-        const int old_c = plan_ncol[a];
+        const int old_c = plan_ncol[a], old_t = plan_threads[a];
         plan_rad[a] = rad;
         plan_ncol[a] = ncol;
+        plan_threads[a] = thr;
         double t = 1e30;
         try {
           build_tables(a);
@@ -759,15 +771,18 @@ struct Solver {
         } else {
           plan_rad[a] = old_r;
           plan_ncol[a] = old_c;
+          plan_threads[a] = old_t;
         }
       };
-      // two rounds: the register family a tile can use depends on both the
-      // radix order and the tile width
+      auto radices = [&] { return plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a]; };
+      // two rounds: the register family a tile can use depends on the radix
+      // order, the tile width and the workgroup size together
       const std::vector<int> cand = contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32, 48}
                                            : std::vector<int>{4, 8, 16, 32};
       for (int round = 0; round < 2; ++round) {
-        for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a]);
-        for (int c : cand) try_cfg(plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a], c);
+        for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a], plan_threads[a]);
+        for (int thr : {256, 512})
+          for (int c : cand) try_cfg(radices(), c, thr);
       }
       build_tables(a);
     }

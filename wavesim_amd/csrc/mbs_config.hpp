@@ -4,13 +4,13 @@
 
 namespace wsim {
 
-#ifndef WSIM_STRIDED_THREADS
-#define WSIM_STRIDED_THREADS 512
-#endif
-// threads per workgroup: 256 for the contiguous-axis passes; the strided
-// passes hold wide tiles (LDS-limited to a few workgroups per CU), so they
-// use larger workgroups to keep more waves resident
-constexpr int threads_for(int contig) { return contig ? 256 : WSIM_STRIDED_THREADS; }
+// workgroup size: every variant is compiled for up to kMaxThreads threads and
+// launched with 256 or 512 (a per-axis plan parameter).  Strided passes hold
+// wide tiles (LDS-limited to a few workgroups per CU) and default to 512 to
+// keep more waves resident; contiguous passes default to 256 (more, smaller
+// workgroups for small grids).
+constexpr int kMaxThreads = 512;
+constexpr int default_threads(int contig) { return contig ? 256 : 512; }
 // registers (complex values) per thread and stage: 32 for scalar fields, 24
 // for the 3-component vector field; the compact family halves them, which
 // lowers the kernel's register count (set by its largest radix path) and

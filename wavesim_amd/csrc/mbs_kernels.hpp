@@ -180,6 +180,7 @@ struct Pass {
   float2* dE_b[C];
   float2* E_b[C];
   const float2* gam_b;
+  uint32_t nt;   // threads of the workgroup (NT is the compile-time maximum)
   ColInfo cfix;  // strided passes: the thread's column never changes
   double eacc;
 
@@ -238,7 +239,7 @@ struct Pass {
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
       int t = 0, j = 0;
       bool ok = b < total;
       if (ok) decode(b, s, t, j);
@@ -254,7 +255,7 @@ struct Pass {
       // wiggle ramps of the first forward pass (WaveSim.m:38-47)
 #pragma unroll
       for (int it = 0; it < ITS; ++it) {
-        const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+        const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
         if (b < total) {
           int t, j;
           decode(b, s, t, j);
@@ -275,7 +276,7 @@ struct Pass {
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
       if (b < total) {
         int t, j;
         decode(b, s, t, j);
@@ -310,7 +311,7 @@ struct Pass {
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
       if (b < total) {
         int t, j;
         decode(b, s, t, j);
@@ -331,7 +332,7 @@ struct Pass {
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
       if (b < total) {
         int t, j;
         decode(b, s, t, j);
@@ -417,7 +418,7 @@ struct Pass {
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * NT;
+      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
       if (b < total) {
         int t, j;
         decode(b, s, t, j);
@@ -507,8 +508,9 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   float2* twf_l = reinterpret_cast<float2*>(smem + off_twf);
   float2* twi_l = reinterpret_cast<float2*>(smem + off_twi);
   float2* lds = reinterpret_cast<float2*>(smem + off_data);
-  for (int i = threadIdx.x; i < ntf; i += NT) twf_l[i] = a.twf[i];
-  for (int i = threadIdx.x; i < nti; i += NT) twi_l[i] = a.twi[i];
+  const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
+  for (int i = threadIdx.x; i < ntf; i += nthr) twf_l[i] = a.twf[i];
+  for (int i = threadIdx.x; i < nti; i += nthr) twi_l[i] = a.twi[i];
 
   const int it_cur = st->it;
   const int nw = st->nw;
@@ -525,12 +527,12 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     vblk = (long long)o * a.n * a.inner + (long long)itile * a.ncol;
   }
 
-  for (int r = threadIdx.x; r < a.n; r += NT) {
+  for (int r = threadIdx.x; r < a.n; r += nthr) {
     ramp_l[r] = a.ramp[a.axis][(size_t)w * a.n + r];
     pe_l[r] = a.pe[a.axis][(size_t)w * a.n + r];
   }
   // per-column coordinates (axes other than a.axis)
-  for (int t = threadIdx.x; t < a.ncol; t += NT) {
+  for (int t = threadIdx.x; t < a.ncol; t += nthr) {
     ColInfo ci;
     int crd[3] = {0, 0, 0};
     int valid;
@@ -580,6 +582,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw,
       st->warm != 0 && a.W0 != nullptr && it_cur <= nw, 1.f / (float)nw, 0u, ramp_l, pe_l};
   p.eacc = 0.0;
+  p.nt = (uint32_t)nthr;
   if (!CONTIG) {
     p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
     p.cfix = cols[threadIdx.x & (a.ncol - 1)];
@@ -695,7 +698,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
       double sum = 0.0;
-      for (int i = 0; i < NT / 64; ++i) sum += red[i];
+      for (int i = 0; i < nthr / 64; ++i) sum += red[i];
       a.partials[(size_t)rhs * a.blocks + bx] = sum;
     }
   }

@@ -10,7 +10,7 @@
 
 namespace wsim {
 KernelFn WSIM_NAME(wsim_kptr, WSIM_CONTIG, WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, WSIM_COMPACT)() {
-  return axis_pass_kernel<(WSIM_CONTIG != 0), WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, threads_for(WSIM_CONTIG),
+  return axis_pass_kernel<(WSIM_CONTIG != 0), WSIM_MODE, WSIM_PRO, WSIM_EPI, WSIM_C, kMaxThreads,
                           ept_for(WSIM_C, WSIM_COMPACT)>;
 }
 }  // namespace wsim
