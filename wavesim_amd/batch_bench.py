@@ -1,7 +1,7 @@
 """Throughput of the two batching modes against one-problem-at-a-time solves.
 
     python -m wavesim_amd.batch_bench ragged [--count 32]   # independent 2-D problems, ragged grids
-    python -m wavesim_amd.batch_bench rhs [--rhs 16 --n 128] # several sources on one 3-D medium
+    python -m wavesim_amd.batch_bench rhs [--rhs 64 --n 192]  # several sources on one 3-D medium
 
 Solves return once their results are final (no extra synchronisation is
 needed around the clocks).  Each mode times the solves only (media, sources and plans set up before the
@@ -106,8 +106,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["ragged", "rhs"])
     ap.add_argument("--count", type=int, default=32)
-    ap.add_argument("--rhs", type=int, default=16)
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--rhs", type=int, default=64)
+    ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--boundary", type=int, default=32)
     ap.add_argument("--reps", type=int, default=2)
     a = ap.parse_args(argv)

@@ -75,13 +75,15 @@ def spheres_3d(n=128, boundary=32, seed=0, residual=1e-3, n_bg=1.33):
     return nmap, opt, Source(np.ones((n, n, 1)), [1, 1, 1])
 
 
-RAGGED_SIZES = (256, 384, 512, 776, 1036, 2048)  # grids 320..2112 with a 32 px boundary
+RAGGED_SIZES = (256, 384, 512, 768, 1024, 2048)  # interiors; 32 px boundary, grids padded to FFT sizes
 
 
 def ragged_2d(count=32, boundary=32, residual=1e-3, sizes=RAGGED_SIZES):
-    """``count`` independent 2-D problems of ragged sizes (cycling through
-    ``sizes``), each its own seeded disordered medium (n = 1 + 0.5 u + 0.01i)
-    with a point source at a seeded position."""
+    """``count`` independent 2-D disordered problems of ragged sizes (cycling
+    through ``sizes``), each its own seeded medium n = 1 + 0.5 u + 0.01i
+    (Gaussian-smoothed, sigma 2 px) with a plane-wave source along x: the
+    ``disordered_2d`` pattern at several sizes.  Grids 768 + 64 and
+    1024 + 64 are padded to 840 and 1100 (SimulationGrid.m:64-80)."""
     out = []
     for i in range(count):
         n = sizes[i % len(sizes)]
@@ -90,8 +92,7 @@ def ragged_2d(count=32, boundary=32, residual=1e-3, sizes=RAGGED_SIZES):
         nmap = 1.0 + 0.5 * u + 0.01j
         opt = dict(pixel_size=0.25, boundary_widths=[boundary, boundary], energy_threshold=residual ** 2,
                    callback_interval=1e9)
-        pos = [int(x) for x in rng.integers(n // 4, 3 * n // 4, size=2)]
-        out.append((nmap, opt, Source(1.0, pos)))
+        out.append((nmap, opt, Source(np.ones(n), [1, 1])))
     return out
 
 
