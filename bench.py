@@ -32,9 +32,14 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "time-to-residual 1e-3 (256^3 c64), learned vs plain; HBM GB/s"
 
 
-def bytes_per_voxel_iteration(ndim: int, C: int) -> int:
-    """Algorithmic HBM bytes per voxel per MBS iteration of the fused pipeline
-    (2D-1 passes; the last reads T, dE', E', gamma and writes dE', E')."""
+def bytes_per_voxel_iteration(ndim: int, C: int, fused: bool = True) -> int:
+    """Algorithmic HBM bytes per voxel per MBS iteration of the pass pipeline.
+    Unfused: 2D-1 passes, 16C each but the last, which reads T, dE', E', gamma
+    and writes dE', E' (40C + 8).  Fused (D >= 2): the last pass also runs the
+    next iteration's first forward pass, writing T (48C + 8), and the first
+    pass disappears: 16C(2D-3) + 48C + 8."""
+    if fused and ndim >= 2:
+        return 16 * C * (2 * ndim - 3) + 48 * C + 8
     return 16 * C * (2 * ndim - 2) + 40 * C + 8
 
 
@@ -204,11 +209,12 @@ def main():
     nvox = int(np.prod(sim.N[:3]))
     C = sim.N[3]
     ndim = sum(1 for n in sim.N[:3] if n > 1)
-    bpv = bytes_per_voxel_iteration(ndim, C)
+    prof = sv.profile(args.profile_reps)
+    fused = any(name == "pass_inv_mix_fwd" for name, _, _ in prof)
+    bpv = bytes_per_voxel_iteration(ndim, C, fused)
     it_per_s = its / t_solve
     hbm_gbps = bpv * nvox * it_per_s / 1e9
 
-    prof = sv.profile(args.profile_reps)
     kern = [p for p in prof if p[2] > 0]
     dom = max(kern, key=lambda p: p[1])
     achieved = dom[2] / (dom[1] * 1e-3) / 1e9
@@ -249,6 +255,9 @@ def main():
         "voxel_updates_per_s": round(C * nvox * it_per_s, 1),
         "hbm_gbps_algorithmic": round(hbm_gbps, 1),
         "iteration_roofline_frac": round(hbm_gbps / HBM_PEAK_GBPS, 4),
+        # the same time against the 5-pass pipeline's bytes (112 B/voxel in 3-D)
+        "iteration_roofline_frac_unfused_bytes": round(bytes_per_voxel_iteration(ndim, C, False) * nvox * it_per_s
+                                                       / 1e9 / HBM_PEAK_GBPS, 4),
         "bytes_per_voxel_iteration": bpv,
         "kernels_us": {n: round(ms * 1e3, 2) for n, ms, _ in prof},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),

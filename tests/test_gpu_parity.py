@@ -197,6 +197,21 @@ def test_long_axes(gpu, shape):
         assert rel_l2(g.propagate(E, w), mbs.propagate(o, E, o.wiggles[w])) <= TOL
 
 
+@pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_3d_disordered])
+def test_fused_first_pass_matches_separate_passes(gpu, builder, monkeypatch):
+    """The last inverse pass running the next iteration's first forward pass
+    (the default) computes the same operations as separate passes; only the
+    compiler's multiply-add contraction may differ (last bits)."""
+    import wavesim_amd as w
+    n, opt, src = builder()
+    E1, s1 = w.WaveSim(n, opt).exec(cases.gpu_source(src))
+    monkeypatch.setenv("WSIM_FUSE", "0")
+    E0, s0 = w.WaveSim(n, opt).exec(cases.gpu_source(src))
+    assert s1.iterations == s0.iterations
+    assert rel_l2(E1, E0) <= 1e-6
+    assert np.allclose(s1.diff_energy, s0.diff_energy, rtol=1e-5)
+
+
 def test_custom_roi(gpu):
     n, opt, src = cases.case_2d_disordered()
     opt = dict(opt, roi=[[5, 7], [30, 44]])

@@ -22,7 +22,7 @@ FNO_LIB = os.path.join(HERE, "libwavesim_fno.so")
 ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
 
 # (contig, mode, prologue, epilogue) -- must match WSIM_FOR_EACH_VARIANT
-VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
+VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 3), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
             (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
 COMPONENTS = (1, 3)
 

@@ -68,7 +68,7 @@ static uint32_t host_fdiv(uint32_t x, const FastDiv& f) {
 
 // radices for a transform of length n: the power-of-two part split as evenly
 // as possible into radix 2..16 stages, then the odd primes 3, 5, 7, 11
-static std::vector<int> default_radices(int n) {
+static std::vector<int> default_radices(int n, int C = 1) {
   std::vector<int> r;
   {
     // experiments/autotuning: WSIM_RADICES_<n>=r0,r1,... overrides the plan
@@ -94,7 +94,8 @@ static std::vector<int> default_radices(int n) {
     ++p2;
   }
   if (p2 > 0) {
-    const int k = (p2 + 3) / 4;
+    const int maxbits = radix_fits(16, C) ? 4 : 3;
+    const int k = (p2 + maxbits - 1) / maxbits;
     for (int i = 0; i < k; ++i) {
       const int bits = p2 / k + (i < p2 % k ? 1 : 0);
       r.push_back(1 << bits);
@@ -253,10 +254,10 @@ static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C, bo
 
 // LDS in front of the transform data: column table and the per-row ramp and
 // coordinate tables (layout of axis_pass_kernel)
-static size_t lds_header(int ncol, int n, int ntw) {
+static size_t lds_header(int ncol, int n, int ntw, bool fuse = false) {
   auto r16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  return r16(ncol * sizeof(ColInfo)) + r16((size_t)n * sizeof(float2)) + r16((size_t)n * sizeof(float)) +
-         (size_t)ntw * sizeof(float2);
+  return r16(ncol * sizeof(ColInfo)) + (fuse ? 2 : 1) * r16((size_t)n * sizeof(float2)) +
+         r16((size_t)n * sizeof(float)) + (size_t)ntw * sizeof(float2);
 }
 
 // contig LDS line pitch: one padding slot every 8 rows, plus one
@@ -323,6 +324,8 @@ struct Solver {
   int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
   int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, 256 or 512 (0: default)
   std::vector<Launch> iter;  // one MBS iteration
+  Launch pre{};                // first forward pass of iteration 1 when fused (fn == nullptr: none)
+  PassArgs* d_pre = nullptr;
   PassArgs* d_args = nullptr;  // device copies of iter[k].args
   std::vector<Launch> probe_iter;
   PassArgs* d_probe_args = nullptr;
@@ -342,7 +345,7 @@ struct Solver {
     gexec = nullptr;
     graph = nullptr;
     void* ptrs[] = {gamma, E, dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
-                    d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args, d_ref, W0};
+                    d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args, d_ref, W0, d_pre};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (int a = 0; a < 3; ++a) {
@@ -367,13 +370,13 @@ struct Solver {
     return plan_threads[axis] > 0 ? plan_threads[axis] : default_threads(contig ? 1 : 0);
   }
 
-  void configure(PassArgs& a, bool contig, int mode, int axis, int& blocks, size_t& lds) {
+  void configure(PassArgs& a, bool contig, int mode, int axis, int& blocks, size_t& lds, bool fuse = false) {
     const AxisPlan& pl = tab[axis].plan;
     const int nthr = threads_of(axis, contig);
     const int n = N[axis];
     a.ntwf = (int)tab[axis].twf.size();
     a.ntwi = (int)tab[axis].twi.size();
-    const int ntw = (mode == MODE_INV ? 0 : a.ntwf) + (mode == MODE_FWD ? 0 : a.ntwi);
+    const int ntw = ((mode == MODE_INV && !fuse) ? 0 : a.ntwf) + (mode == MODE_FWD ? 0 : a.ntwi);
     a.axis = axis;
     a.n = n;
     a.inner = 1;
@@ -388,7 +391,7 @@ struct Solver {
         bytes = (size_t)C * ncol * contig_pitch(n) * sizeof(float2);
       else
         bytes = (size_t)n * (C * ncol + 1) * sizeof(float2);
-      bytes += lds_header(ncol, n, ntw);
+      bytes += lds_header(ncol, n, ntw, fuse);
       return bytes <= lds_cap;
     };
     // long axes whose single-column tile exceeds the target take up to the
@@ -437,7 +440,7 @@ struct Solver {
       lds = (size_t)C * a.ncol * contig_pitch(n) * sizeof(float2);
     else
       lds = (size_t)n * (C * a.ncol + 1) * sizeof(float2);
-    lds += lds_header(a.ncol, n, ntw);
+    lds += lds_header(a.ncol, n, ntw, fuse);
   }
 
   PassArgs base_args() const {
@@ -467,7 +470,7 @@ struct Solver {
     L.args.twf = d_twf[axis];
     L.args.twi = d_twi[axis];
     int blocks = 0;
-    configure(L.args, contig, mode, axis, blocks, L.lds);
+    configure(L.args, contig, mode, axis, blocks, L.lds, epi == EPI_MIX_FWD);
     // the compact register family whenever the tile fits it (WSIM_COMPACT=0 disables)
     const char* env_k = std::getenv("WSIM_COMPACT");
     const bool compact = !(env_k && std::atoi(env_k) == 0) && tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, true, threads_of(axis, contig));
@@ -479,7 +482,7 @@ struct Solver {
     L.args.grp_n = 1;
     L.args.grp_end = nullptr;
     L.name = name;
-    const double perv = (epi == EPI_MIX) ? (40.0 * C + 8.0) : 16.0 * C;
+    const double perv = epi == EPI_MIX ? 40.0 * C + 8.0 : epi == EPI_MIX_FWD ? 48.0 * C + 8.0 : 16.0 * C;
     L.bytes = perv * (double)nvox * nrhs;
     if (L.lds > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -487,9 +490,19 @@ struct Solver {
     return L;
   }
 
+  // the last inverse pass also runs the next iteration's first forward pass
+  // (same axis, same lines) when that axis has at least two FFT stages;
+  // WSIM_FUSE=0 keeps them apart
+  bool fuse_first_pass() const {
+    const char* e = std::getenv("WSIM_FUSE");
+    if (e && std::atoi(e) == 0) return false;
+    return axes.size() >= 2 && tab[axes[0]].plan.nst >= 2;
+  }
+
   // the launches of one iteration (in = dE', out = T, T in place, T -> E', dE')
   void build_iteration(std::vector<Launch>& out, bool probe_mode) {
     out.clear();
+    if (!probe_mode) pre = Launch{};
     const int D = (int)axes.size();
     const int epi = probe_mode ? EPI_RAMP : EPI_MIX;
     const float2* in0 = probe_mode ? probe : dE;
@@ -527,7 +540,9 @@ struct Solver {
         L.args.out = T;
         out.push_back(L);
       }
-      Launch LE = make_launch(true, MODE_INV, PRO_NONE, epi, axes[0], "pass_inv_mix");
+      const bool fuse = !probe_mode && fuse_first_pass();
+      Launch LE = make_launch(true, MODE_INV, PRO_NONE, fuse ? EPI_MIX_FWD : epi, axes[0],
+                              fuse ? "pass_inv_mix_fwd" : "pass_inv_mix");
       LE.args.in = T;
       LE.args.out = probe_mode ? probe : T;
       LE.args.dE = dE;
@@ -535,7 +550,14 @@ struct Solver {
       LE.args.gamma = gamma;
       LE.args.partials = partials;
       LE.args.W0 = probe_mode ? nullptr : W0;
+      LE.args.S = fuse ? S : nullptr;
       out.push_back(LE);
+      if (fuse) {
+        // the first forward pass runs once per solve (before iteration 1);
+        // every later one is the tail of the previous iteration's last pass
+        pre = out.front();
+        out.erase(out.begin());
+      }
     }
     // tile order: consecutive passes walk memory in opposite directions, so
     // each pass starts on what its predecessor wrote last (still in the
@@ -593,7 +615,7 @@ struct Solver {
 
   // twiddle tables of axis a for its current radix plan
   void build_tables(int a) {
-    std::vector<int> rad = plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a];
+    std::vector<int> rad = plan_rad[a].empty() ? default_radices(N[a], C) : plan_rad[a];
     if (N[a] == 1) rad = {};
     tab[a] = build_axis(N[a], rad);
     if (d_twf[a]) (void)hipFree(d_twf[a]);
@@ -622,6 +644,13 @@ struct Solver {
     d_args = nullptr;
     iter.swap(it2);
     upload_args(iter, &d_args);
+    if (d_pre) (void)hipFree(d_pre);
+    d_pre = nullptr;
+    if (pre.fn) {
+      std::vector<Launch> one{pre};
+      upload_args(one, &d_pre);
+      pre.dargs = d_pre;
+    }
     probe_iter.clear();
     if (d_probe_args) (void)hipFree(d_probe_args);
     d_probe_args = nullptr;
@@ -693,6 +722,8 @@ struct Solver {
         const int x = std::atoi(rs.c_str() + q);
         require(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 11 || x == 16,
                 "unsupported radix in plan '" + item + "'");
+        require(radix_fits(x, C), "radix " + std::to_string(x) + " is not built for " + std::to_string(C) +
+                                      "-component fields in plan '" + item + "'");
         rad.push_back(x);
         prod *= x;
         while (q < rs.size() && rs[q] != ',') ++q;
@@ -712,7 +743,7 @@ struct Solver {
 
   // every factorisation of n into supported radices with at most `extra`
   // stages more than the fewest, in every distinct order (capped)
-  static std::vector<std::vector<int>> radix_candidates(int n, int extra, size_t cap) {
+  static std::vector<std::vector<int>> radix_candidates(int n, int extra, size_t cap, int C) {
     static const int R[] = {16, 8, 4, 2, 11, 7, 5, 3};
     std::vector<std::vector<int>> sets;
     std::vector<int> cur;
@@ -723,7 +754,7 @@ struct Solver {
       }
       if (cur.size() >= (size_t)MAXST) return;
       for (int i = start; i < 8; ++i)
-        if (rem % R[i] == 0) {
+        if (rem % R[i] == 0 && radix_fits(R[i], C)) {
           cur.push_back(R[i]);
           rec(rem / R[i], i);
           cur.pop_back();
@@ -774,13 +805,13 @@ struct Solver {
           plan_threads[a] = old_t;
         }
       };
-      auto radices = [&] { return plan_rad[a].empty() ? default_radices(N[a]) : plan_rad[a]; };
+      auto radices = [&] { return plan_rad[a].empty() ? default_radices(N[a], C) : plan_rad[a]; };
       // two rounds: the register family a tile can use depends on the radix
       // order, the tile width and the workgroup size together
       const std::vector<int> cand = contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32, 48}
                                            : std::vector<int>{4, 8, 16, 32};
       for (int round = 0; round < 2; ++round) {
-        for (const auto& rad : radix_candidates(N[a], 1, 48)) try_cfg(rad, plan_ncol[a], plan_threads[a]);
+        for (const auto& rad : radix_candidates(N[a], 1, 48, C)) try_cfg(rad, plan_ncol[a], plan_threads[a]);
         for (int thr : {256, 512})
           for (int c : cand) try_cfg(radices(), c, thr);
       }
@@ -852,7 +883,7 @@ struct Solver {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // tables
     for (int a = 0; a < 3; ++a) {
-      require(N[a] == 1 || !default_radices(N[a]).empty(),
+      require(N[a] == 1 || !default_radices(N[a], C).empty(),
               "axis length " + std::to_string(N[a]) + " has a prime factor above 11",
               WSIM_ERR_UNSUPPORTED);
       build_tables(a);
@@ -1025,6 +1056,8 @@ struct Solver {
     } else {
       HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
     }
+    // fused iterations: the first forward pass of iteration 1 runs here
+    if (pre.fn) launch(pre);
     HIPCHK(hipStreamSynchronize(stream));
   }
 

@@ -21,10 +21,11 @@ using KernelFn = void (*)(const PassArgs*);
 
 // the pass variants the solver launches, as numbers (contig, mode, prologue,
 // epilogue) with mode 0 fwd / 1 inv / 2 mid, prologue 1 = source + ramps,
-// epilogue 1 = mix, 2 = ramps only
+// epilogue 1 = mix, 2 = ramps only, 3 = mix + the next iteration's first pass
 #define WSIM_FOR_EACH_VARIANT(X) \
   X(1, 0, 1, 0)                  \
   X(1, 1, 0, 1)                  \
+  X(1, 1, 0, 3)                  \
   X(1, 1, 0, 2)                  \
   X(1, 2, 1, 1)                  \
   X(1, 2, 1, 2)                  \

@@ -13,6 +13,11 @@
 //                 and accumulate (private/f_mix.m:3-4, WaveSimBase.m:179-180),
 //                 and the per-workgroup ROI energy partial sums
 //                 (Simulation.m:265-273)
+//   EPI_MIX_FWD   as EPI_MIX, then the next iteration's first forward pass on
+//                 the same lines: dE' (+ the next step's source / warm-start
+//                 portion) times the next wiggle's ramps goes to LDS and
+//                 through the forward FFT stages to the work field.  The
+//                 first forward pass then runs only once, before iteration 1.
 //
 // Layout: a field is [rhs][comp][z][x][y] complex64 (MATLAB column-major
 // [y,x,z,pol], y contiguous).  A pass along axis `a` sees each component as
@@ -86,7 +91,7 @@ struct DevState {
 
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
 enum { PRO_NONE = 0, PRO_SRC_RAMP = 1 };
-enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2 };
+enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2, EPI_MIX_FWD = 3 };
 
 struct PassArgs {
   int N[3];  // Ny Nx Nz
@@ -132,6 +137,10 @@ struct PassArgs {
                        // previous pass wrote last, still in the memory-side cache)
 };
 
+// radix R is compiled for C-component fields when the R*C values of one
+// butterfly fit the register budget
+__host__ __device__ constexpr bool radix_fits(int R, int C) { return R * C <= 33; }
+
 // butterflies per thread per stage: EPT registers' worth, at most kMaxIts
 // (more unrolled iterations of the small radices only add register pressure)
 constexpr int kMaxIts = 6;
@@ -142,6 +151,7 @@ __host__ __device__ constexpr int its_per_stage(int ept, int C, int R) {
 struct ColInfo {
   uint32_t off;  // element offset of row 0 from the workgroup's base (same for every component)
   float2 rc;     // product of the other axes' ramps
+  float2 rc2;    // the same for the next iteration's wiggle (EPI_MIX_FWD)
   float p[3];    // other axes' coordinates (axis entry unused)
   float q;       // sum of squares of the other axes' coordinates
   int valid;
@@ -172,6 +182,9 @@ struct Pass {
   float inv_nw;
   uint32_t lg_ncol;
   const float2* ramp_a;  // this wiggle's ramp along the axis
+  const float2* ramp2_a; // the next iteration's (EPI_MIX_FWD)
+  bool add_src2;         // the next iteration adds the source (EPI_MIX_FWD)
+  bool warm_step2;       // the next iteration adds the warm-start portion
   const float* pe_a;
   const float2* in_b[C];  // workgroup bases, per component
   float2* out_b[C];
@@ -270,7 +283,7 @@ struct Pass {
     }
   }
 
-  template <int R>
+  template <int R, bool FIRST = false>
   __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, const float2* tw) const {
     constexpr int ITS = its<R>();
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
@@ -287,7 +300,7 @@ struct Pass {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float2 x = lds[lidx(t, c, j + r * s.nb)];
-            v[(c * ITS + it) * R + r] = r == 0 ? x : cmul(x, twk[r - 1]);
+            v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[r - 1]);
           }
       } else {
 #pragma unroll
@@ -325,8 +338,10 @@ struct Pass {
     }
   }
 
-  // last stage: Ns = n / R, outputs land on rows j + r*nb
-  template <int R>
+  // last stage: Ns = n / R, outputs land on rows j + r*nb; E2 = the
+  // epilogue applied on the way out (EPI, or EPI_NONE for the forward tail of
+  // EPI_MIX_FWD)
+  template <int R, int E2 = EPI>
   __device__ __forceinline__ void stage_store_global(const float2* v, const Stage& s) {
     constexpr int ITS = its<R>();
     const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
@@ -338,15 +353,15 @@ struct Pass {
         decode(b, s, t, j);
         const ColInfo& ci = col(t);
         if (!ci.valid) continue;
-        if (EPI == EPI_MIX) {
-          mix_rows<R>(v, it, j, s.nb, ci);
+        if (E2 == EPI_MIX || E2 == EPI_MIX_FWD) {
+          mix_rows<R>(v, it, t, j, s.nb, ci);
           continue;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
           const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
-          if (EPI == EPI_NONE) {
+          if (E2 == EPI_NONE) {
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
           } else {  // EPI_RAMP: undo the ramps only (propagation probe)
@@ -365,9 +380,13 @@ struct Pass {
   // a load after a store to a possibly aliasing address would wait for the
   // previous row's whole load-compute-store chain.
   template <int R>
-  __device__ __forceinline__ void mix_rows(const float2* v, int it, int j, int nb, const ColInfo& ci) {
+  __device__ __forceinline__ void mix_rows(const float2* v, int it, int t, int j, int nb, const ColInfo& ci) {
     constexpr int ITS = its<R>();
-    constexpr int CH = WSIM_EPI_CHUNK < R ? WSIM_EPI_CHUNK : R;
+    // rows per chunk: 4 for scalar fields (2 when the next forward pass is
+    // fused, whose operand hand-off holds more registers), 1 per component
+    // for vector fields
+    constexpr int CH0 = C == 1 ? (EPI == EPI_MIX_FWD ? WSIM_EPI_CHUNK / 2 : WSIM_EPI_CHUNK) : 1;
+    constexpr int CH = CH0 < R ? CH0 : R;
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += CH) {
@@ -406,6 +425,16 @@ struct Pass {
           stg(E_b[c], i, cadd(e0[q][c], dn));
           if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
             eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
+          if (EPI == EPI_MIX_FWD) {
+            // the next iteration's first-pass operand, as load_raw and the
+            // ramp step of the first forward pass would form it
+            // (the source is read here, not with the other operands: it is
+            // needed in the first Nw iterations only and would hold registers)
+            float2 x = dn;
+            if (warm_step2) x = cadd(x, w0[q][c]);
+            if (add_src2) x = cadd(x, ldg(S_b[c], i));
+            lds[lidx(t, c, row)] = cmul(x, cmul(ramp2_a[row], ci.rc2));
+          }
         }
       }
     }
@@ -457,17 +486,21 @@ struct Pass {
 #define WSIM_RADIX_SWITCH(RV, ...) \
   { constexpr int R = WSIM_ONLY_RADIX; __VA_ARGS__ }
 #else
-#define WSIM_RADIX_SWITCH(RV, ...)                        \
-  switch (RV) {                                           \
-    case 2: { constexpr int R = 2; __VA_ARGS__ } break;   \
-    case 3: { constexpr int R = 3; __VA_ARGS__ } break;   \
-    case 4: { constexpr int R = 4; __VA_ARGS__ } break;   \
-    case 5: { constexpr int R = 5; __VA_ARGS__ } break;   \
-    case 7: { constexpr int R = 7; __VA_ARGS__ } break;   \
-    case 8: { constexpr int R = 8; __VA_ARGS__ } break;   \
-    case 11: { constexpr int R = 11; __VA_ARGS__ } break; \
-    case 16: { constexpr int R = 16; __VA_ARGS__ } break; \
-    default: break;                                       \
+// radices whose register tile a C-component field can hold (16 is left out
+// for 3-component fields: 48 values per thread spill)
+#define WSIM_RADIX_CASE(RR, ...) \
+  case RR: { constexpr int R = RR; if constexpr (radix_fits(RR, C)) { __VA_ARGS__ } } break;
+#define WSIM_RADIX_SWITCH(RV, ...)          \
+  switch (RV) {                             \
+    WSIM_RADIX_CASE(2, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(3, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(4, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(5, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(7, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(8, __VA_ARGS__)         \
+    WSIM_RADIX_CASE(11, __VA_ARGS__)        \
+    WSIM_RADIX_CASE(16, __VA_ARGS__)        \
+    default: break;                         \
   }
 #endif
 
@@ -494,16 +527,20 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   const DevState* st = a.st;
   if (st->stopped[rhs]) return;
 
-  // LDS: [ColInfo x ncol][ramp_a x n][pe_a x n][transform data]
+  // LDS: [ColInfo x ncol][ramp_a x n][(ramp2 x n)][pe_a x n][twiddles][transform data]
+  constexpr bool FUSE = EPI == EPI_MIX_FWD;
   ColInfo* cols = reinterpret_cast<ColInfo*>(smem);
+  const size_t rn = ((size_t)a.n * sizeof(float2) + 15) & ~(size_t)15;
   const size_t off_ramp = (a.ncol * sizeof(ColInfo) + 15) & ~(size_t)15;
-  const size_t off_pe = off_ramp + (((size_t)a.n * sizeof(float2) + 15) & ~(size_t)15);
-  const int ntf = MODE == MODE_INV ? 0 : a.ntwf;  // forward twiddles used by this pass
+  const size_t off_ramp2 = off_ramp + rn;
+  const size_t off_pe = off_ramp + (FUSE ? 2 : 1) * rn;
+  const int ntf = (MODE == MODE_INV && !FUSE) ? 0 : a.ntwf;  // forward twiddles used by this pass
   const int nti = MODE == MODE_FWD ? 0 : a.ntwi;  // inverse twiddles used by this pass
   const size_t off_twf = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
   const size_t off_twi = off_twf + (size_t)ntf * sizeof(float2);
   const size_t off_data = off_twi + (size_t)nti * sizeof(float2);
   float2* ramp_l = reinterpret_cast<float2*>(smem + off_ramp);
+  float2* ramp2_l = reinterpret_cast<float2*>(smem + off_ramp2);
   float* pe_l = reinterpret_cast<float*>(smem + off_pe);
   float2* twf_l = reinterpret_cast<float2*>(smem + off_twf);
   float2* twi_l = reinterpret_cast<float2*>(smem + off_twi);
@@ -515,6 +552,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   const int it_cur = st->it;
   const int nw = st->nw;
   const int w = (it_cur - 1) % nw;
+  const int w2 = it_cur % nw;  // the next iteration's wiggle (EPI_MIX_FWD)
 
   // workgroup geometry: voxel base and the decomposition of block index
   long long vblk;
@@ -529,6 +567,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
 
   for (int r = threadIdx.x; r < a.n; r += nthr) {
     ramp_l[r] = a.ramp[a.axis][(size_t)w * a.n + r];
+    if (FUSE) ramp2_l[r] = a.ramp[a.axis][(size_t)w2 * a.n + r];
     pe_l[r] = a.pe[a.axis][(size_t)w * a.n + r];
   }
   // per-column coordinates (axes other than a.axis)
@@ -558,7 +597,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
         if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
     }
     ci.valid = valid;
-    float2 rc = make_float2(1.f, 0.f);
+    float2 rc = make_float2(1.f, 0.f), rc2 = make_float2(1.f, 0.f);
     float q = 0.f;
     int roi_ok = 1;
 #pragma unroll
@@ -569,9 +608,11 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       ci.p[b] = pb;
       q += pb * pb;
       rc = cmul(rc, a.ramp[b][(size_t)w * a.N[b] + crd[b]]);
+      if (FUSE) rc2 = cmul(rc2, a.ramp[b][(size_t)w2 * a.N[b] + crd[b]]);
       roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
     }
     ci.rc = rc;
+    ci.rc2 = rc2;
     ci.q = q;
     ci.roi_ok = roi_ok;
     cols[t] = ci;
@@ -580,7 +621,9 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
 
   using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT>;
   P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw,
-      st->warm != 0 && a.W0 != nullptr && it_cur <= nw, 1.f / (float)nw, 0u, ramp_l, pe_l};
+      st->warm != 0 && a.W0 != nullptr && it_cur <= nw, 1.f / (float)nw, 0u, ramp_l, ramp2_l,
+      FUSE && a.S != nullptr && it_cur + 1 <= nw, FUSE && st->warm != 0 && a.W0 != nullptr && it_cur + 1 <= nw,
+      pe_l};
   p.eacc = 0.0;
   p.nt = (uint32_t)nthr;
   if (!CONTIG) {
@@ -651,9 +694,42 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       WSIM_RADIX_SWITCH(sl.R,
         float2 v[P::template its<R>() * C * R];
         p.template stage_load_lds<R>(v, sl, tw);
+        if (FUSE) __syncthreads();  // the epilogue writes the next pass's operand to LDS
         p.template stage_compute<R, D0>(v);
         p.template stage_store_global<R>(v, sl);
       )
+      if (FUSE) {
+        // the next iteration's forward transform of the same lines: LDS ->
+        // forward stages -> work field (the plan has at least two stages)
+        __syncthreads();
+        const Stage* ff = pl.f;
+        WSIM_RADIX_SWITCH(ff[0].R,
+          float2 v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R, true>(v, ff[0], twf_l);
+          __syncthreads();
+          p.template stage_compute<R, -1>(v);
+          p.template stage_store_lds<R>(v, ff[0]);
+        )
+        __syncthreads();
+        for (int s = 1; s < S - 1; ++s) {
+          const Stage st_ = ff[s];
+          WSIM_RADIX_SWITCH(st_.R,
+            float2 v[P::template its<R>() * C * R];
+            p.template stage_load_lds<R>(v, st_, twf_l);
+            __syncthreads();
+            p.template stage_compute<R, -1>(v);
+            p.template stage_store_lds<R>(v, st_);
+          )
+          __syncthreads();
+        }
+        const Stage fl = ff[S - 1];
+        WSIM_RADIX_SWITCH(fl.R,
+          float2 v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R>(v, fl, twf_l);
+          p.template stage_compute<R, -1>(v);
+          p.template stage_store_global<R, EPI_NONE>(v, fl);
+        )
+      }
     } else {
       // forward last stage, Green's function, inverse first stage (same
       // radix, Ns = 1, no twiddles), back to LDS
@@ -688,7 +764,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     }
   }
 
-  if (EPI == EPI_MIX) {
+  if (EPI == EPI_MIX || EPI == EPI_MIX_FWD) {
     // block reduction of the ROI energy (fixed order: deterministic)
     __shared__ double red[NT / 64];
     double e = p.eacc;
