@@ -32,6 +32,7 @@ _PASS = {  # (CONTIG, MODE, PRO, EPI) -> pass name
     ("false", "1", "0", "0"): "pass_inv",
     ("true", "1", "0", "1"): "pass_inv_mix",
     ("true", "1", "0", "2"): "pass_inv_mix",
+    ("true", "1", "0", "3"): "pass_inv_mix_fwd",
     ("true", "2", "1", "1"): "pass_mid_pro_epi",
     ("true", "2", "1", "2"): "pass_mid_pro_epi",
 }
