@@ -135,3 +135,20 @@ def test_warmstart_preprocessing_kernels(gpu):
     ref = O.compose(grid, lo, roi, 2, P.cpu().numpy().view(np.complex64).reshape(n.shape), A, uf, foot)
     got = E0.cpu().numpy().view(np.complex64).reshape(grid, order="F")
     assert rel_l2(got, ref) <= 1e-6
+
+
+def test_warmstart_forward_graph_replays_match_eager(gpu):
+    """The captured forward graph gives the eager forward's output on every
+    replay, for changing inputs, with eager work in between."""
+    import torch
+    from wavesim_amd.learned.fno import Config
+    from wavesim_amd.learned.warmstart import CIN, WarmStart
+    ws = WarmStart(Config(L=16, cin=CIN, width=8, modes=4, layers=2, proj=16), factor=2, seed=3)
+    rng = np.random.default_rng(0)
+    for k in range(4):
+        X = torch.from_numpy(rng.standard_normal(CIN * 16 ** 3).astype(np.float32)).cuda()
+        ref = ws.forward(X, graph=False).clone()
+        torch.cuda.synchronize()
+        got = ws.forward(X, graph=True).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref), k
