@@ -212,6 +212,27 @@ def test_fused_first_pass_matches_separate_passes(gpu, builder, monkeypatch):
     assert np.allclose(s1.diff_energy, s0.diff_energy, rtol=1e-5)
 
 
+@pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_3d_disordered])
+def test_propagate_matches_rocfft(gpu, builder):
+    """The hand-written FFT passes against rocFFT: the same wiggle ramps and
+    scalar Green's function (tables from the oracle) around torch.fft.fftn /
+    ifftn on the GPU, in complex64."""
+    import torch
+    n, opt, src = builder()
+    o, g = _pair(n, opt)
+    rng = np.random.default_rng(8)
+    E = (rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)).astype(np.complex64)
+    for w, wig in enumerate(o.wiggles):
+        t = lambda a, shape: torch.from_numpy(np.asarray(a).astype(np.complex64).reshape(shape)).cuda()
+        ramp = t(wig["gy"], (-1, 1, 1, 1)) * t(wig["gx"], (1, -1, 1, 1)) * t(wig["gz"], (1, 1, -1, 1))
+        den = ((wig["pxe"] ** 2).reshape(1, -1, 1, 1) + (wig["pye"] ** 2).reshape(-1, 1, 1, 1)
+               + (wig["pze"] ** 2 - o.k02e).reshape(1, 1, -1, 1))
+        x = torch.from_numpy(E).cuda() * ramp
+        x = torch.fft.ifftn(torch.fft.fftn(x, dim=(0, 1, 2)) / t(den, den.shape), dim=(0, 1, 2))
+        ref = (x * ramp.conj()).cpu().numpy()
+        assert rel_l2(g.propagate(E, w), ref) <= TOL
+
+
 def test_custom_roi(gpu):
     n, opt, src = cases.case_2d_disordered()
     opt = dict(opt, roi=[[5, 7], [30, 44]])
