@@ -808,8 +808,9 @@ struct Solver {
       auto radices = [&] { return plan_rad[a].empty() ? default_radices(N[a], C) : plan_rad[a]; };
       // two rounds: the register family a tile can use depends on the radix
       // order, the tile width and the workgroup size together
-      const std::vector<int> cand = contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32, 48}
-                                           : std::vector<int>{4, 8, 16, 32};
+      // (narrow tiles give small grids enough workgroups to fill the CUs)
+      const std::vector<int> cand = contig ? std::vector<int>{1, 2, 4, 6, 8, 12, 16, 24, 32, 48}
+                                           : std::vector<int>{1, 2, 4, 8, 16, 32};
       for (int round = 0; round < 2; ++round) {
         for (const auto& rad : radix_candidates(N[a], 1, 48, C)) try_cfg(rad, plan_ncol[a], plan_threads[a]);
         for (int thr : {256, 512})
