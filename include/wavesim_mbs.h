@@ -126,14 +126,19 @@ int wsim_solver_device_ptr(wsim_solver* s, int which, int rhs, void** dptr);
  * WaveSimVector.m:19-51) to a host field; in and out hold N[0..3] entries. */
 int wsim_solver_propagate(wsim_solver* s, int wiggle, const wsim_cfloat* in, wsim_cfloat* out);
 /* Average duration (ms) of each kernel of one iteration, timed with HIP events
- * on the solver's stream over `reps` iterations; names[i] gets a static string. */
+ * on the solver's stream over `reps` iterations; bytes[i] = its algorithmic HBM
+ * bytes per launch; names[i] gets a static string.  With D >= 2 axes the last
+ * pass also runs the next iteration's first forward pass ("pass_inv_mix_fwd")
+ * and the separate first pass runs only once per solve (not listed). */
 int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, double* bytes,
                         const char** names, int* nkernels);
-/* FFT plan of every non-trivial axis, as text "y:8,8,5/16;x:4,16,5/16;..."
+/* FFT plan of every non-trivial axis, as text "y:5,4,16/16/512;x:5,16,4/16;..."
  * (radices of the Stockham stages in forward order / workgroup columns,
- * 0 = default).  wsim_solver_tune times whole iterations of this problem for
- * candidate radix orders and tile widths, axis by axis, keeps the fastest and
- * reports it (the reference picks FFT-friendly sizes only,
+ * 0 = widest that fits / optional workgroup size, 256 or 512).  Radices are
+ * 2, 3, 4, 5, 7, 8, 11, 16 (16 not for 3-component fields).
+ * wsim_solver_tune times whole iterations of this problem for candidate radix
+ * orders, tile widths and workgroup sizes, axis by axis, keeps the fastest
+ * and reports it (the reference picks FFT-friendly sizes only,
  * SimulationGrid.m:64-80; the per-size plan choice has no counterpart). */
 int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
 int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);
