@@ -233,6 +233,22 @@ def test_propagate_matches_rocfft(gpu, builder):
         assert rel_l2(g.propagate(E, w), ref) <= TOL
 
 
+def test_1d_against_reference_analytic_solution(gpu):
+    """The GPU solve of a 1-D homogeneous line with ARL + ACC against the
+    reference's utilities/homogeneous_medium_analytic_solution.m (restated in
+    oracle.analytic), in complex64 to an energy ratio of 1e-10."""
+    import wavesim_amd as w
+    from oracle import analytic
+    Ny = 256
+    sim = w.WaveSim(np.ones((Ny, 1)), dict(pixel_size=0.25, boundary_widths=[32, 0], energy_threshold=1e-10))
+    E, st = sim.exec(w.Source(1.0, [Ny // 2, 1]))
+    x = (np.arange(Ny) - (Ny // 2 - 1)) * 0.25
+    ref = analytic.homogeneous_1d(sim.k0, 0.25, x)
+    e = rel_l2(np.asarray(E).reshape(-1), ref)
+    print(f"1-D analytic rel-L2 {e:.2e} after {st.iterations} iterations", flush=True)
+    assert st.converged and e < 2e-5  # the complex64 oracle reaches 7.0e-6 here
+
+
 def test_custom_roi(gpu):
     n, opt, src = cases.case_2d_disordered()
     opt = dict(opt, roi=[[5, 7], [30, 44]])
