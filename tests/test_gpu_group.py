@@ -80,3 +80,28 @@ def test_group_rejects_duplicates(gpu):
     sv = probs[0][0].solver(1)
     with pytest.raises(WsimError):
         w.SolverGroup([sv, sv])
+
+
+def test_group_with_warm_start_equals_single(gpu):
+    """A warm-started member (the start is staged in the first pass before the
+    grouped iterations) gives its single solve's result."""
+    import wavesim_amd as w
+    specs, probs = _problems()
+    (sim0, src0), (sim1, src1) = probs[1], probs[3]
+    rng = np.random.default_rng(4)
+    E0 = (0.5 * rng.standard_normal(sim0.N) + 0.5j * rng.standard_normal(sim0.N)).astype(np.complex64)
+    svs = []
+    for sim, src in ((sim0, src0), (sim1, src1)):
+        p = sim._prepare(src)
+        sv = sim.solver(1)
+        sv.set_source(0, p.positions, p.values)
+        svs.append(sv)
+    svs[0].set_warm_start(0, E0)
+    svs[0].solve(sim0.max_iterations(), sim0.energy_threshold)
+    single = svs[0].field(0).copy()
+    it_single = svs[0].result(0).iterations
+    with w.SolverGroup(svs) as g:
+        g.solve([s.max_iterations() for s in (sim0, sim1)], [s.energy_threshold for s in (sim0, sim1)])
+    assert svs[0].result(0).iterations == it_single
+    assert np.array_equal(svs[0].field(0), single)
+    svs[0].clear_warm_start()
