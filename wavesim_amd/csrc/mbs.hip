@@ -137,8 +137,10 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
       st[s].tw = (int)tw.size();
       st[s].fNs = make_fastdiv((uint32_t)Ns);
       st[s].fnb = make_fastdiv((uint32_t)(n / R));
-      for (int k = 0; k < Ns; ++k)
-        for (int r = 1; r < R; ++r) {
+      // [r-1][k] order: the butterflies of one wave have consecutive k, so
+      // their twiddle reads from LDS hit consecutive banks
+      for (int r = 1; r < R; ++r)
+        for (int k = 0; k < Ns; ++k) {
           const double ang = sign * 2.0 * M_PI * (double)r * (double)k / ((double)Ns * R);
           tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
         }

@@ -294,13 +294,13 @@ struct Pass {
         int t, j;
         decode(b, s, t, j);
         const int k = j - (int)fdiv((uint32_t)j, s.fNs) * s.Ns;
-        const float2* twk = tw + s.tw + k * (R - 1);
+        const float2* twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float2 x = lds[lidx(t, c, j + r * s.nb)];
-            v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[r - 1]);
+            v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[(r - 1) * s.Ns]);
           }
       } else {
 #pragma unroll
