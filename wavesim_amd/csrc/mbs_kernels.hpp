@@ -160,12 +160,28 @@ struct ColInfo {
 
 // 32-bit element offsets from a workgroup-uniform base pointer, so the
 // compiler can use SGPR-base + 32-bit VGPR-offset addressing
+// The field pointers are read from the argument block in device memory, so
+// the compiler cannot infer that they address global memory and would emit
+// flat instructions -- which also count against the LDS wait counter, making
+// every LDS wait of a stage wait for the outstanding HBM traffic too.  Casting
+// to the global address space gives global_load / global_store with a
+// scalar base.
+using gptr_c = const __attribute__((address_space(1))) char*;
+using gptr = __attribute__((address_space(1))) char*;
+using v2f = float __attribute__((ext_vector_type(2)));  // plain vector type: no constructors in address space 1
 __device__ __forceinline__ float2 ldg(const float2* base, uint32_t i) {
-  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + (size_t)(i * 8u));
+  const v2f x = *reinterpret_cast<const __attribute__((address_space(1))) v2f*>((gptr_c)base + (size_t)(i * 8u));
+  return make_float2(x.x, x.y);
 }
 __device__ __forceinline__ void stg(float2* base, uint32_t i, float2 v) {
-  *reinterpret_cast<float2*>(reinterpret_cast<char*>(base) + (size_t)(i * 8u)) = v;
+  const v2f x = {v.x, v.y};
+  *reinterpret_cast<__attribute__((address_space(1))) v2f*>((gptr)base + (size_t)(i * 8u)) = x;
 }
+// one element of a table in global memory
+__device__ __forceinline__ float ldt(const float* p) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)p);
+}
+__device__ __forceinline__ float2 ldt(const float2* p) { return ldg(p, 0); }
 
 #ifndef WSIM_EPI_CHUNK  // rows of one column whose mix operands are loaded together
 #define WSIM_EPI_CHUNK 4
@@ -204,6 +220,23 @@ struct Pass {
     return CONTIG ? (c * a.ncol + t) * a.lds_pitch + row + (row >> 3) : row * a.lds_pitch + c * a.ncol + t;
   }
 
+  // HBM element index of row `row` of a column whose row 0 is at `off`
+  // (contiguous lines: unit stride)
+  __device__ __forceinline__ uint32_t gidx(uint32_t off, int row) const {
+    return CONTIG ? off + (uint32_t)row : off + (uint32_t)row * (uint32_t)a.rstride;
+  }
+  // the step of gidx for rows d apart (uniform)
+  __device__ __forceinline__ uint32_t gstep(int d) const {
+    return CONTIG ? (uint32_t)d : (uint32_t)d * (uint32_t)a.rstride;
+  }
+  // LDS index of rows row0 + r*d: contiguous lines pad every 8 rows, so their
+  // index is computed per row; strided tiles are row-major, so the index of
+  // row0 is computed once and each further row adds d * pitch (the row
+  // products were the bulk of the integer work of a stage)
+  __device__ __forceinline__ int lrow(int t, int c, int row0, int d, int r) const {
+    return CONTIG ? lidx(t, c, row0 + r * d) : lidx(t, c, row0) + r * (d * a.lds_pitch);
+  }
+
   // butterfly b -> (column t, butterfly index j)
   __device__ __forceinline__ void decode(uint32_t b, const Stage& s, int& t, int& j) const {
     if (CONTIG) {
@@ -221,8 +254,7 @@ struct Pass {
   // ---- element transforms --------------------------------------------------
   // first-stage operand before the ramps: dE' (+ W0/Nw in the warm steps,
   // + i/eps/Nw * S in the source steps)
-  __device__ __forceinline__ float2 load_raw(uint32_t off, int c, int row) const {
-    const uint32_t i = off + (uint32_t)row * (uint32_t)a.rstride;
+  __device__ __forceinline__ float2 load_raw(uint32_t i, int c) const {
     float2 v = ldg(in_b[c], i);
     if (PRO == PRO_SRC_RAMP) {
       if (warm_step) v = cadd(v, cscale(ldg(W0_b[c], i), inv_nw));
@@ -258,11 +290,12 @@ struct Pass {
       if (ok) decode(b, s, t, j);
       const ColInfo& ci = col(t);
       ok = ok && ci.valid;
+      const uint32_t g0 = gidx(ci.off, j), gs = gstep(s.nb);
 #pragma unroll
       for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          v[(c * ITS + it) * R + r] = ok ? load_raw(ci.off, c, j + r * s.nb) : make_float2(0.f, 0.f);
+          v[(c * ITS + it) * R + r] = ok ? load_raw(g0 + (uint32_t)r * gs, c) : make_float2(0.f, 0.f);
     }
     if (PRO == PRO_SRC_RAMP) {
       // wiggle ramps of the first forward pass (WaveSim.m:38-47)
@@ -299,7 +332,7 @@ struct Pass {
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const float2 x = lds[lidx(t, c, j + r * s.nb)];
+            const float2 x = lds[lrow(t, c, j, s.nb, r)];
             v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[(r - 1) * s.Ns]);
           }
       } else {
@@ -333,7 +366,7 @@ struct Pass {
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[lidx(t, c, base + r * s.Ns)] = v[(c * ITS + it) * R + r];
+          for (int r = 0; r < R; ++r) lds[lrow(t, c, base, s.Ns, r)] = v[(c * ITS + it) * R + r];
       }
     }
   }
@@ -357,10 +390,11 @@ struct Pass {
           mix_rows<R>(v, it, t, j, s.nb, ci);
           continue;
         }
+        const uint32_t g0 = gidx(ci.off, j), gs = gstep(s.nb);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
-          const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+          const uint32_t i = g0 + (uint32_t)r * gs;
           if (E2 == EPI_NONE) {
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
@@ -394,7 +428,7 @@ struct Pass {
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
         if (r0 + q >= R) break;
-        const uint32_t i = ci.off + (uint32_t)(j + (r0 + q) * nb) * (uint32_t)a.rstride;
+        const uint32_t i = gidx(ci.off, j + (r0 + q) * nb);
         gm[q] = ldg(gam_b, i);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -408,7 +442,7 @@ struct Pass {
         if (r0 + q >= R) break;
         const int r = r0 + q;
         const int row = j + r * nb;
-        const uint32_t i = ci.off + (uint32_t)row * (uint32_t)a.rstride;
+        const uint32_t i = gidx(ci.off, row);
         const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
         const float2 c1 = make_float2(1.f - 0.7f * gm[q].x, -0.7f * gm[q].y);
         const float2 g2 = cmul(gm[q], gm[q]);
@@ -546,8 +580,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   float2* twi_l = reinterpret_cast<float2*>(smem + off_twi);
   float2* lds = reinterpret_cast<float2*>(smem + off_data);
   const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
-  for (int i = threadIdx.x; i < ntf; i += nthr) twf_l[i] = a.twf[i];
-  for (int i = threadIdx.x; i < nti; i += nthr) twi_l[i] = a.twi[i];
+  for (int i = threadIdx.x; i < ntf; i += nthr) twf_l[i] = ldt(a.twf + i);
+  for (int i = threadIdx.x; i < nti; i += nthr) twi_l[i] = ldt(a.twi + i);
 
   const int it_cur = st->it;
   const int nw = st->nw;
@@ -566,9 +600,9 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
 
   for (int r = threadIdx.x; r < a.n; r += nthr) {
-    ramp_l[r] = a.ramp[a.axis][(size_t)w * a.n + r];
-    if (FUSE) ramp2_l[r] = a.ramp[a.axis][(size_t)w2 * a.n + r];
-    pe_l[r] = a.pe[a.axis][(size_t)w * a.n + r];
+    ramp_l[r] = ldt(a.ramp[a.axis] + (size_t)w * a.n + r);
+    if (FUSE) ramp2_l[r] = ldt(a.ramp[a.axis] + (size_t)w2 * a.n + r);
+    pe_l[r] = ldt(a.pe[a.axis] + (size_t)w * a.n + r);
   }
   // per-column coordinates (axes other than a.axis)
   for (int t = threadIdx.x; t < a.ncol; t += nthr) {
@@ -604,11 +638,11 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     for (int b = 0; b < 3; ++b) {
       ci.p[b] = 0.f;
       if (b == a.axis) continue;
-      const float pb = a.pe[b][(size_t)w * a.N[b] + crd[b]];
+      const float pb = ldt(a.pe[b] + (size_t)w * a.N[b] + crd[b]);
       ci.p[b] = pb;
       q += pb * pb;
-      rc = cmul(rc, a.ramp[b][(size_t)w * a.N[b] + crd[b]]);
-      if (FUSE) rc2 = cmul(rc2, a.ramp[b][(size_t)w2 * a.N[b] + crd[b]]);
+      rc = cmul(rc, ldt(a.ramp[b] + (size_t)w * a.N[b] + crd[b]));
+      if (FUSE) rc2 = cmul(rc2, ldt(a.ramp[b] + (size_t)w2 * a.N[b] + crd[b]));
       roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
     }
     ci.rc = rc;
