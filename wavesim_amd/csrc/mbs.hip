@@ -432,6 +432,11 @@ struct Solver {
       while (ncol > 1 && ncol / 2 >= a.inner) ncol >>= 1;
       a.ncol = ncol;
       a.lds_pitch = C * ncol + 1;
+      for (int q = 0; q < a.plan.nst; ++q)
+        for (Stage* st : {&a.plan.f[q], &a.plan.i[q]}) {
+          st->nbp = st->nb * a.lds_pitch;
+          st->nsp = st->Ns * a.lds_pitch;
+        }
       a.ntile_inner = (a.inner + ncol - 1) / ncol;
       a.ftile = make_fastdiv((uint32_t)a.ntile_inner);
       const long long outer = nvox / ((long long)a.inner * n);

@@ -56,8 +56,9 @@ struct Stage {
   int R;   // radix
   int Ns;  // product of the radices of the earlier stages
   int nb;  // n / R butterflies per line
-  int tw;  // offset of this stage's twiddles: tw[k*(R-1) + r-1], k < Ns, 1 <= r < R
+  int tw;  // offset of this stage's twiddles: tw[(r-1)*Ns + k], k < Ns, 1 <= r < R
   FastDiv fNs, fnb;
+  int nbp, nsp;  // strided tiles: nb and Ns rows in LDS words of float2 (times the pitch)
 };
 
 struct AxisPlan {
@@ -233,8 +234,10 @@ struct Pass {
   // index is computed per row; strided tiles are row-major, so the index of
   // row0 is computed once and each further row adds d * pitch (the row
   // products were the bulk of the integer work of a stage)
-  __device__ __forceinline__ int lrow(int t, int c, int row0, int d, int r) const {
-    return CONTIG ? lidx(t, c, row0 + r * d) : lidx(t, c, row0) + r * (d * a.lds_pitch);
+  // (dp = d * pitch, precomputed on the host: the compiler would otherwise
+  // fold the products back into one multiply per row)
+  __device__ __forceinline__ int lrow(int t, int c, int row0, int d, int dp, int r) const {
+    return CONTIG ? lidx(t, c, row0 + r * d) : lidx(t, c, row0) + r * dp;
   }
 
   // butterfly b -> (column t, butterfly index j)
@@ -332,7 +335,7 @@ struct Pass {
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const float2 x = lds[lrow(t, c, j, s.nb, r)];
+            const float2 x = lds[lrow(t, c, j, s.nb, s.nbp, r)];
             v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[(r - 1) * s.Ns]);
           }
       } else {
@@ -366,7 +369,7 @@ struct Pass {
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[lrow(t, c, base, s.Ns, r)] = v[(c * ITS + it) * R + r];
+          for (int r = 0; r < R; ++r) lds[lrow(t, c, base, s.Ns, s.nsp, r)] = v[(c * ITS + it) * R + r];
       }
     }
   }
