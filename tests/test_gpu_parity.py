@@ -155,6 +155,23 @@ def test_max_iterations_cap(gpu):
     assert st.iterations == 20 and not st.converged
 
 
+@pytest.mark.parametrize("max_iterations", [6, 7])
+def test_last_increment_on_odd_iteration(gpu, max_iterations):
+    """E' takes its increments two at a time, on even iterations; a solve that
+    stops after an odd iteration adds its last increment at the end.  With
+    Nw = 1 (periodic, no boundary layers) a stop can fall on any iteration;
+    single solves, multi-RHS solves and groups all finish E'."""
+    import wavesim_amd as w
+    n, opt, src = cases.case_periodic_2d(thr=1e-30)
+    o, g, Eo, so, Eg, sg = _run_both(n, opt, src, max_iterations=max_iterations)
+    assert sg.iterations == max_iterations
+    _check_solve(Eo, so, Eg, sg)
+    [(Em, sm)] = w.exec_many([(w.WaveSim(n, opt), cases.gpu_source(src))], max_iterations=max_iterations)
+    assert sm.iterations == max_iterations and np.array_equal(Em, Eg)
+    Eb, sb = g.exec_batch([cases.gpu_source(src)] * 2, max_iterations=max_iterations)
+    assert all(np.array_equal(e, Eg) for e in Eb)
+
+
 def test_zero_source(gpu):
     import wavesim_amd as w
     n, opt, _ = cases.case_2d_homogeneous()

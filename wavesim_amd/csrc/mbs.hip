@@ -239,6 +239,20 @@ __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2*
     x[i] = cmul(x[i], g[i % nvox]);
 }
 
+// The mix epilogue adds the increments to E' two at a time, on even
+// iterations; a rhs whose last iteration was odd still lacks its last dE'.
+// One block row per rhs; blocks of a rhs that needs nothing return at once.
+__global__ void __launch_bounds__(256) finish_e_kernel(const DevState* st, float2* E, const float2* dE,
+                                                      long long fe) {
+  const int r = blockIdx.y;
+  const int last = st->stopped[r] ? st->n_iter[r] : st->it - 1;
+  if ((last & 1) == 0) return;
+  float2* e = E + (size_t)r * fe;
+  const float2* d = dE + (size_t)r * fe;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < fe; i += (long long)gridDim.x * 256)
+    e[i] = cadd(e[i], d[i]);
+}
+
 // ----------------------------------------------------------------------------
 // kernel table
 // ----------------------------------------------------------------------------
@@ -489,7 +503,10 @@ struct Solver {
     L.args.grp_n = 1;
     L.args.grp_end = nullptr;
     L.name = name;
-    const double perv = epi == EPI_MIX ? 40.0 * C + 8.0 : epi == EPI_MIX_FWD ? 48.0 * C + 8.0 : 16.0 * C;
+    // algorithmic bytes per voxel, averaged over an even and an odd
+    // iteration (E' is read and written on even iterations only): the mix
+    // passes read T, gamma, dE' (+ E') and write dE' (+ E') (+ the next T)
+    const double perv = epi == EPI_MIX ? 32.0 * C + 8.0 : epi == EPI_MIX_FWD ? 40.0 * C + 8.0 : 16.0 * C;
     L.bytes = perv * (double)nvox * nrhs;
     if (L.lds > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1126,9 +1143,20 @@ struct Solver {
         }
       }
     }
+    finish_e();
     HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
     last_iters = host_state.it - 1;
     return rc;
+  }
+
+  // E' of every rhs complete (finish_e_kernel), ordered on the solver's stream
+  void finish_e() {
+    const long long fe = field_elems();
+    const unsigned bx = (unsigned)std::min<long long>((fe + 255) / 256, 4096);
+    hipLaunchKernelGGL(finish_e_kernel, dim3(bx, (unsigned)nrhs), dim3(256), 0, stream, (const DevState*)dst, E,
+                       (const float2*)dE, fe);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   void result(int rhs, int64_t* n_iter, int* conv, double* diff, double* eall, int64_t cap) {
@@ -1405,6 +1433,7 @@ struct Group {
       HIPCHK(hipMemcpy(done.data(), d_done, done.size() * sizeof(int), hipMemcpyDeviceToHost));
     }
     for (Solver* p : ps) {
+      p->finish_e();
       HIPCHK(hipMemcpy(&p->host_state, p->dst, sizeof(DevState), hipMemcpyDeviceToHost));
       p->last_iters = p->host_state.it - 1;
     }

@@ -178,6 +178,18 @@ __device__ __forceinline__ void stg(float2* base, uint32_t i, float2 v) {
   const v2f x = {v.x, v.y};
   *reinterpret_cast<__attribute__((address_space(1))) v2f*>((gptr)base + (size_t)(i * 8u)) = x;
 }
+// A pointer the whole workgroup shares, moved to scalar registers.  Values
+// computed with 64-bit multiplies land in vector registers even when every
+// lane holds the same one, and a vector base costs each access a 64-bit
+// address add; a scalar base lets global_load / global_store take it with
+// the 32-bit lane offset directly.
+template <typename T>
+__device__ __forceinline__ T* wave_uniform(T* p) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
 // one element of a table in global memory
 __device__ __forceinline__ float ldt(const float* p) {
   return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)p);
@@ -213,12 +225,21 @@ struct Pass {
   uint32_t nt;   // threads of the workgroup (NT is the compile-time maximum)
   ColInfo cfix;  // strided passes: the thread's column never changes
   double eacc;
+  // E' += dE' is carried out on even iterations only, as E' += dE'_{k-1} +
+  // dE'_k: the mix reads dE'_{k-1} and forms dE'_k anyway, so E' is read and
+  // written every other iteration.  (E' + dE'_{k-1}) + dE'_k rounds exactly
+  // as the two separate updates; a solve that ends on an odd iteration adds
+  // its last dE' afterwards (finish_e_kernel).
+  bool acc_e;
 
   // contig lines get one padding slot every 8 rows: the Stockham stores of
   // the early stages (Ns < 8) write rows R apart, which would otherwise land
   // on the same LDS banks
+  // (24-bit products: LDS indices stay far below 2^24, and a 24-bit multiply
+  // issues at full rate where a 32-bit one takes four cycles)
   __device__ __forceinline__ int lidx(int t, int c, int row) const {
-    return CONTIG ? (c * a.ncol + t) * a.lds_pitch + row + (row >> 3) : row * a.lds_pitch + c * a.ncol + t;
+    return CONTIG ? (int)__umul24((uint32_t)(c * a.ncol + t), (uint32_t)a.lds_pitch) + row + (row >> 3)
+                  : (int)__umul24((uint32_t)row, (uint32_t)a.lds_pitch) + c * a.ncol + t;
   }
 
   // HBM element index of row `row` of a column whose row 0 is at `off`
@@ -254,6 +275,60 @@ struct Pass {
 
   __device__ __forceinline__ const ColInfo& col(int t) const { return CONTIG ? cols[t] : cfix; }
 
+  // The butterflies of a stage that this thread takes are b = tid + it*nt,
+  // it = 0, 1, ...  Rather than dividing every b, a cursor starts from one
+  // division and steps by nt = dt*nb + dj (contiguous lines) or by dj rows
+  // (strided tiles: nt is a multiple of ncol, so the column never changes).
+  // k = j mod Ns steps by dk = dj mod Ns (nb is a multiple of Ns, so the
+  // wrap of j past nb leaves k alone); g is the HBM index of row j (strided).
+  struct Cursor {
+    int t, j, k;
+    uint32_t g;
+  };
+  struct Walk {
+    int dt, dj, dk;
+    uint32_t dg;
+  };
+  __device__ __forceinline__ Walk walk(const Stage& s) const {
+    Walk w;
+    if (CONTIG) {
+      w.dt = (int)fdiv(nt, s.fnb);
+      w.dj = (int)nt - w.dt * s.nb;
+    } else {
+      w.dt = 0;
+      w.dj = (int)(nt >> lg_ncol);
+    }
+    w.dk = w.dj - (int)fdiv((uint32_t)w.dj, s.fNs) * s.Ns;
+    w.dg = (uint32_t)w.dj * (uint32_t)a.rstride;
+    return w;
+  }
+  __device__ __forceinline__ Cursor start(const Stage& s) const {
+    Cursor c;
+    decode((uint32_t)tid, s, c.t, c.j);
+    c.k = c.j - (int)fdiv((uint32_t)c.j, s.fNs) * s.Ns;
+    c.g = CONTIG ? 0u : cfix.off + (uint32_t)c.j * (uint32_t)a.rstride;
+    return c;
+  }
+  // b < ncol * nb
+  __device__ __forceinline__ bool live(const Cursor& c, const Stage& s) const {
+    return CONTIG ? c.t < a.ncol : c.j < s.nb;
+  }
+  __device__ __forceinline__ void step(Cursor& c, const Walk& w, const Stage& s) const {
+    c.t += w.dt;
+    c.j += w.dj;
+    c.k += w.dk;
+    c.g += w.dg;
+    if (CONTIG && c.j >= s.nb) {
+      c.j -= s.nb;
+      ++c.t;
+    }
+    if (c.k >= s.Ns) c.k -= s.Ns;
+  }
+  // HBM index of row j of the cursor's column
+  __device__ __forceinline__ uint32_t gbase(const Cursor& c, const ColInfo& ci) const {
+    return CONTIG ? ci.off + (uint32_t)c.j : c.g;
+  }
+
   // ---- element transforms --------------------------------------------------
   // first-stage operand before the ramps: dE' (+ W0/Nw in the warm steps,
   // + i/eps/Nw * S in the source steps)
@@ -284,37 +359,36 @@ struct Pass {
   template <int R>
   __device__ __forceinline__ void stage_load_global(float2* v, const Stage& s) const {
     constexpr int ITS = its<R>();
-    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const Walk w = walk(s);
+    Cursor cu = start(s);
+    const uint32_t gs = gstep(s.nb);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-      int t = 0, j = 0;
-      bool ok = b < total;
-      if (ok) decode(b, s, t, j);
-      const ColInfo& ci = col(t);
-      ok = ok && ci.valid;
-      const uint32_t g0 = gidx(ci.off, j), gs = gstep(s.nb);
+      const bool lv = live(cu, s);
+      const ColInfo& ci = col(lv ? cu.t : 0);
+      const bool ok = lv && ci.valid;
+      const uint32_t g0 = gbase(cu, ci);
 #pragma unroll
       for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r)
           v[(c * ITS + it) * R + r] = ok ? load_raw(g0 + (uint32_t)r * gs, c) : make_float2(0.f, 0.f);
+      step(cu, w, s);
     }
     if (PRO == PRO_SRC_RAMP) {
       // wiggle ramps of the first forward pass (WaveSim.m:38-47)
+      cu = start(s);
 #pragma unroll
       for (int it = 0; it < ITS; ++it) {
-        const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-        if (b < total) {
-          int t, j;
-          decode(b, s, t, j);
-          const float2 rc = col(t).rc;
+        if (live(cu, s)) {
+          const float2 rc = col(cu.t).rc;
 #pragma unroll
           for (int c = 0; c < C; ++c)
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              v[(c * ITS + it) * R + r] = cmul(v[(c * ITS + it) * R + r], cmul(ramp_a[j + r * s.nb], rc));
+              v[(c * ITS + it) * R + r] = cmul(v[(c * ITS + it) * R + r], cmul(ramp_a[cu.j + r * s.nb], rc));
         }
+        step(cu, w, s);
       }
     }
   }
@@ -322,20 +396,17 @@ struct Pass {
   template <int R, bool FIRST = false>
   __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, const float2* tw) const {
     constexpr int ITS = its<R>();
-    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const Walk w = walk(s);
+    Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-      if (b < total) {
-        int t, j;
-        decode(b, s, t, j);
-        const int k = j - (int)fdiv((uint32_t)j, s.fNs) * s.Ns;
-        const float2* twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
+      if (live(cu, s)) {
+        const float2* twk = tw + s.tw + cu.k;  // table [r-1][k]: lanes with consecutive k read consecutive words
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const float2 x = lds[lrow(t, c, j, s.nb, s.nbp, r)];
+            const float2 x = lds[lrow(cu.t, c, cu.j, s.nb, s.nbp, r)];
             v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[(r - 1) * s.Ns]);
           }
       } else {
@@ -344,6 +415,7 @@ struct Pass {
 #pragma unroll
           for (int r = 0; r < R; ++r) v[(c * ITS + it) * R + r] = make_float2(0.f, 0.f);
       }
+      step(cu, w, s);
     }
   }
 
@@ -357,20 +429,18 @@ struct Pass {
   template <int R>
   __device__ __forceinline__ void stage_store_lds(const float2* v, const Stage& s) const {
     constexpr int ITS = its<R>();
-    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const Walk w = walk(s);
+    Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-      if (b < total) {
-        int t, j;
-        decode(b, s, t, j);
-        const int k = j - (int)fdiv((uint32_t)j, s.fNs) * s.Ns;
-        const int base = (j - k) * R + k;
+      if (live(cu, s)) {
+        const int base = (cu.j - cu.k) * R + cu.k;
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[lrow(t, c, base, s.Ns, s.nsp, r)] = v[(c * ITS + it) * R + r];
+          for (int r = 0; r < R; ++r) lds[lrow(cu.t, c, base, s.Ns, s.nsp, r)] = v[(c * ITS + it) * R + r];
       }
+      step(cu, w, s);
     }
   }
 
@@ -380,20 +450,20 @@ struct Pass {
   template <int R, int E2 = EPI>
   __device__ __forceinline__ void stage_store_global(const float2* v, const Stage& s) {
     constexpr int ITS = its<R>();
-    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const Walk w = walk(s);
+    Cursor cu = start(s);
+    const uint32_t gs = gstep(s.nb);
 #pragma unroll
-    for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-      if (b < total) {
-        int t, j;
-        decode(b, s, t, j);
+    for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
+      if (live(cu, s)) {
+        const int t = cu.t, j = cu.j;
         const ColInfo& ci = col(t);
         if (!ci.valid) continue;
         if (E2 == EPI_MIX || E2 == EPI_MIX_FWD) {
           mix_rows<R>(v, it, t, j, s.nb, ci);
           continue;
         }
-        const uint32_t g0 = gidx(ci.off, j), gs = gstep(s.nb);
+        const uint32_t g0 = gbase(cu, ci);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
@@ -436,7 +506,7 @@ struct Pass {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           d0[q][c] = ldg(dE_b[c], i);
-          e0[q][c] = ldg(E_b[c], i);
+          e0[q][c] = acc_e ? ldg(E_b[c], i) : make_float2(0.f, 0.f);
           w0[q][c] = warm_step ? cscale(ldg(W0_b[c], i), inv_nw) : make_float2(0.f, 0.f);
         }
       }
@@ -459,7 +529,7 @@ struct Pass {
           const float2 w = w0[q][c];
           const float2 dn = csub(csub(cmul(c1, cadd(d0[q][c], w)), cmul(c2, ep)), w);
           stg(dE_b[c], i, dn);
-          stg(E_b[c], i, cadd(e0[q][c], dn));
+          if (acc_e) stg(E_b[c], i, cadd(cadd(e0[q][c], d0[q][c]), dn));
           if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
             eacc += (double)dn.x * (double)dn.x + (double)dn.y * (double)dn.y;
           if (EPI == EPI_MIX_FWD) {
@@ -481,14 +551,13 @@ struct Pass {
   template <int R>
   __device__ __forceinline__ void stage_green(float2* v, const Stage& s) const {
     constexpr int ITS = its<R>();
-    const uint32_t total = (uint32_t)a.ncol * (uint32_t)s.nb;
+    const Walk w = walk(s);
+    Cursor cu = start(s);
 #pragma unroll
-    for (int it = 0; it < ITS; ++it) {
-      const uint32_t b = (uint32_t)tid + (uint32_t)it * nt;
-      if (b < total) {
-        int t, j;
-        decode(b, s, t, j);
-        const ColInfo& ci = col(t);
+    for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
+      if (live(cu, s)) {
+        const int j = cu.j;
+        const ColInfo& ci = col(cu.t);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
@@ -662,6 +731,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       FUSE && a.S != nullptr && it_cur + 1 <= nw, FUSE && st->warm != 0 && a.W0 != nullptr && it_cur + 1 <= nw,
       pe_l};
   p.eacc = 0.0;
+  p.acc_e = (it_cur & 1) == 0;
   p.nt = (uint32_t)nthr;
   if (!CONTIG) {
     p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
@@ -671,14 +741,14 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const long long cb = rbase + (long long)c * a.nvox;
-    p.in_b[c] = a.in + cb;
-    p.out_b[c] = a.out + cb;
-    p.S_b[c] = a.S + cb;
-    p.W0_b[c] = a.W0 + cb;
-    p.dE_b[c] = a.dE + cb;
-    p.E_b[c] = a.E + cb;
+    p.in_b[c] = wave_uniform(a.in + cb);
+    p.out_b[c] = wave_uniform(a.out + cb);
+    p.S_b[c] = wave_uniform(a.S + cb);
+    p.W0_b[c] = wave_uniform(a.W0 + cb);
+    p.dE_b[c] = wave_uniform(a.dE + cb);
+    p.E_b[c] = wave_uniform(a.E + cb);
   }
-  p.gam_b = a.gamma + vblk;
+  p.gam_b = wave_uniform(a.gamma + vblk);
 
   const AxisPlan& pl = a.plan;
   const int S = pl.nst;
