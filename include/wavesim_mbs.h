@@ -73,8 +73,11 @@ typedef int (*wsim_callback)(const char* type, int64_t it, const double* diff_en
  *            column c in [y x z pol] (Simulation.m:117-124)
  *  E_out     N[0]*N[1]*N[2]*N[3] complex: the primed field E' = gamma*E on the
  *            full grid; the caller divides by gamma and crops (WaveSimBase.m:150-153)
- *  diff_energy_out  one entry per iteration (capacity diff_energy_cap);
- *            *n_diff_energy receives the number of iterations run.
+ *  diff_energy_out  one entry per iteration (capacity diff_energy_cap; entries
+ *            past it are not written); *n_diff_energy receives the number of
+ *            iterations run.  The cap max_iter applies at the first multiple of
+ *            Nwiggles at or after it (Simulation.m:202), so up to
+ *            ceil(max_iter / nwiggles) * nwiggles iterations run.
  * The loop follows WaveSimBase.m:157-197 exactly: source i/eps/Nw added during
  * the first Nw iterations, alpha = 0.7, energy refreshed at iterations
  * 1, 2, 2+interval, ... with interval = min(8, callback_interval)

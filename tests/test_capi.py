@@ -35,6 +35,22 @@ def test_library_exports_every_declared_symbol():
     assert set(capi.SYMBOLS) >= set(_declared_symbols())
 
 
+def test_mex_shim_compiles_against_the_header():
+    """The MEX shim (built on a MATLAB host) against include/wavesim_mbs.h and
+    declarations of the MATLAB C API it calls: a syntax check, so that a change
+    of the C ABI cannot leave the shim behind unnoticed."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "tests", "mex_stub"),
+                        "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "wavesim_amd", "mex", "RunWaveSimAlgorithmMex.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_selftest_and_version():
     from wavesim_amd import capi
     L = capi.lib()

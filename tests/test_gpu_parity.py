@@ -172,6 +172,16 @@ def test_last_increment_on_odd_iteration(gpu, max_iterations):
     assert all(np.array_equal(e, Eg) for e in Eb)
 
 
+def test_iteration_cap_waits_for_a_full_wiggle_cycle(gpu):
+    """max_iterations takes effect at the first multiple of Nw at or after it
+    (Simulation.m:202): a fresh solver capped at 18 with Nw = 4 runs 20
+    iterations, as the oracle does."""
+    n, opt, src = cases.case_2d_homogeneous(thr=1e-30)
+    o, g, Eo, so, Eg, sg = _run_both(n, opt, src, max_iterations=18)
+    assert so.it - 1 == 20
+    _check_solve(Eo, so, Eg, sg)
+
+
 def test_zero_source(gpu):
     import wavesim_amd as w
     n, opt, _ = cases.case_2d_homogeneous()

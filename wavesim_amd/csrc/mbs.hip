@@ -1029,6 +1029,14 @@ struct Solver {
     HIPCHK(hipMemcpy(d_ref + rhs, &e, sizeof(double), hipMemcpyHostToDevice));
   }
 
+  // history length for a solve capped at max_iter: the cap applies at the
+  // first multiple of Nw at or after it (Simulation.m:202), so the history
+  // (whose end also stops the loop) must reach that iteration
+  int history_for(long long max_iter) const {
+    const long long cap = std::min<long long>(max_iter, 2000000);
+    return (int)(((cap + nw - 1) / nw) * nw + 1);
+  }
+
   void ensure_history(int cap) {
     if (cap <= hist_cap) return;
     if (d_diff) (void)hipFree(d_diff);
@@ -1102,8 +1110,7 @@ struct Solver {
   int solve(long long max_iter, double thr, long long batch, const char* cbtype, double cbint,
             wsim_callback cb, void* user) {
     require(max_iter >= 1, "max_iter must be >= 1");
-    const long long cap = std::min<long long>(max_iter, 2000000);
-    ensure_history((int)cap + 1);
+    ensure_history(history_for(max_iter));
     reset_state(max_iter, thr);
     if (batch <= 0) {
       // about 2 ms of work per batch, a multiple of Nw
@@ -1403,7 +1410,7 @@ struct Group {
       Solver* p = ps[i];
       require(max_iter[i] >= 1, "max_iter must be >= 1");
       if (p->iter.empty() || !p->d_args) p->rebuild_launches();
-      p->ensure_history((int)std::min<long long>(max_iter[i], 2000000) + 1);
+      p->ensure_history(p->history_for(max_iter[i]));
       p->reset_state(max_iter[i], thr[i]);
       nw = std::max(nw, p->nw);
       it_us += 112.0 * (double)p->nvox * p->C * p->nrhs / 5e6;

@@ -104,7 +104,11 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
 
   const mwSize dims[4] = {(mwSize)N[0], (mwSize)N[1], (mwSize)N[2], (mwSize)N[3]};
   plhs[0] = mxCreateNumericArray(4, dims, mxSINGLE_CLASS, mxCOMPLEX);
-  const int64_t cap = (int64_t)std::min(max_iter, 1e7);
+  // diff_energy has one entry per iteration run: the cap (max_iter, or
+  // 100000 when it is not a positive number, WaveSimBase.m:127-131) takes
+  // effect at the first multiple of Nwiggles at or after it (Simulation.m:202)
+  const double mi = (std::isfinite(max_iter) && max_iter > 0) ? std::floor(max_iter) : 100000.0;
+  const int64_t cap = (int64_t)std::min(std::ceil(mi / std::max(nw, 1)) * std::max(nw, 1), 1e7);
   std::vector<double> de((size_t)cap);
   int64_t n = 0;
   const int rc = wsim_run_algorithm(N, gamma, nw, wig.data(), (int)np, patches.data(), eps,
@@ -114,6 +118,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                     de.data(), cap, &n);
   if (rc != WSIM_OK && rc != WSIM_ERR_ABORTED)
     mexErrMsgIdAndTxt("wsim:run", "wsim_run_algorithm failed (%d): %s", rc, wsim_last_error());
+  n = std::min(n, cap);
   plhs[1] = mxCreateDoubleMatrix(1, (mwSize)n, mxREAL);
   std::memcpy(mxGetDoubles(plhs[1]), de.data(), (size_t)n * sizeof(double));
 }
