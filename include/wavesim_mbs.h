@@ -187,6 +187,66 @@ int wsim_group_create(wsim_group** out, wsim_solver* const* solvers, int n);
 int wsim_group_solve(wsim_group* g, const int64_t* max_iter, const double* energy_threshold, int64_t batch);
 int wsim_group_destroy(wsim_group* g);
 
+/* ---------------------------------------------------------------------------
+ * Problem set-up without MATLAB or Python: the constructor of WaveSim /
+ * WaveSimVector (WaveSimBase.m:40-82 with Medium.m:15-72, SimulationGrid.m:
+ * 20-80, apply_edge_filters WaveSimBase.m:217-258, compute_wiggles /
+ * wiggle_descriptor WaveSimBase.m:272-316, wiggle_perm.m:1-29), from a
+ * refractive-index map to what wsim_solver_create takes, with the reference's
+ * double-precision arithmetic (results cast to single as data_array does,
+ * Simulation.m:221-252).  Only the ARL boundary (the reference's default;
+ * the PBL types are deprecated, Medium.m:86).
+ *
+ * wsim_problem_query gives the grid, ROI and wiggle count; the caller then
+ * allocates gamma (N[0]*N[1]*N[2] values, column-major [y x z]) and, per
+ * wiggle, pxe/gx with N[1], pye/gy with N[0] and pze/gz with N[2] entries, and
+ * wsim_problem_build fills them and the rest of info.  The result of
+ * build is the solver_desc fields N, nwiggles, epsilon, k02e and roi. */
+typedef struct {
+  int64_t shape[3];           /* refractive-index map [ny nx nz] (1 for absent axes) */
+  double pixel_size;          /* opt.pixel_size */
+  double lambda;              /* opt.lambda (0: the reference's default 1) */
+  int64_t boundary_widths[3]; /* opt.boundary_widths in pixels; 0 = periodic axis */
+  double epsilon;             /* > 0: forced (opt.epsilon); 0: calculate_epsilon */
+  double epsilonmin;          /* opt.epsilonmin (0: 3) */
+  int acc[3];                 /* ACC per axis (1/0); all three -1: on for the non-periodic axes */
+  int vector;                 /* 0: WaveSim, 1: WaveSimVector (3 field components) */
+  int has_roi;                /* 1: roi below, 1-based inclusive [y x z pol] within the map */
+  int64_t roi[2][4];
+} wsim_problem_desc;
+
+typedef struct {
+  int64_t N[4];               /* grid [Ny Nx Nz Npol] */
+  int64_t roi[2][4];          /* 1-based inclusive within the grid */
+  int64_t Bl[3], Br[3];       /* padding before / after the map per axis (Medium.m:67-68) */
+  int nwiggles;
+  float epsilon;              /* single precision, as the loop uses it */
+  wsim_cfloat k02e;           /* k0c^2/epsilon + i */
+  double k0c;
+  double iterations_per_cycle;
+} wsim_problem_info;
+
+/* Wiggle tables to be filled (the same arrays as wsim_wiggle, writable). */
+typedef struct {
+  float* pxe;
+  float* pye;
+  float* pze;
+  wsim_cfloat* gx;
+  wsim_cfloat* gy;
+  wsim_cfloat* gz;
+} wsim_wiggle_out;
+
+typedef struct {
+  double re, im;
+} wsim_cdouble;
+
+int wsim_problem_query(const wsim_problem_desc* desc, wsim_problem_info* info);
+/* n: the refractive-index map (shape[0]*shape[1]*shape[2] complex values in
+ * double precision, as MATLAB holds it, column-major); imaginary parts must
+ * be >= 0 (no gain, Medium.m:44). */
+int wsim_problem_build(const wsim_problem_desc* desc, const wsim_cdouble* n, wsim_problem_info* info,
+                       wsim_cfloat* gamma, wsim_wiggle_out* wiggles);
+
 const char* wsim_last_error(void);
 int wsim_abi_version(void);
 /* Host-side self checks of the index arithmetic the kernels rely on. */

@@ -182,6 +182,31 @@ def test_iteration_cap_waits_for_a_full_wiggle_cycle(gpu):
     _check_solve(Eo, so, Eg, sg)
 
 
+@pytest.mark.parametrize("builder,vector", [(cases.case_2d_disordered, False), (cases.case_3d_disordered, True)])
+def test_problem_built_by_the_library_solves_like_exec(gpu, builder, vector):
+    """A solver created from wsim_problem_build's outputs (the C++ set-up, no
+    Python mirror) gives exec's field and energies bit for bit."""
+    import wavesim_amd as w
+    from wavesim_amd import problem
+    from wavesim_amd.solver import MBSSolver
+    n, opt, src = builder()
+    sim = (w.WaveSimVector if vector else w.WaveSim)(n, opt)
+    Eg, sg = sim.exec(cases.gpu_source(src))
+    p = problem.build(n, opt, vector=vector)
+    sv = MBSSolver(p["N"], p["gamma"], p["wiggles"], float(p["epsilon"]), complex(p["k02e"]), p["roi"],
+                   sim.energy_calculation_interval)
+    ps = sim._prepare(cases.gpu_source(src))
+    sv.set_source(0, ps.positions, ps.values)
+    sv.solve(sim.max_iterations(), sim.energy_threshold)
+    r = sv.result(0)
+    assert r.iterations == sg.iterations
+    assert np.array_equal(r.diff_energy, sg.diff_energy)
+    with np.errstate(invalid="ignore", divide="ignore"):  # gamma is 0 at the grid's outer edge
+        E = sim.crop_field(sv.field(0) / np.asarray(sim.gamma)[..., None])
+    assert np.array_equal(E, Eg)
+    sv.close()
+
+
 def test_zero_source(gpu):
     import wavesim_amd as w
     n, opt, _ = cases.case_2d_homogeneous()

@@ -52,7 +52,8 @@ def _run(cmd):
 
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), [])]
+    units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), []),
+             (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
         for c in COMPONENTS:
             for k in (0, 1):

@@ -43,6 +43,22 @@ class desc_t(C.Structure):
                 ("energy_interval", C.c_int)]
 
 
+class cdouble(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class problem_desc_t(C.Structure):
+    _fields_ = [("shape", C.c_int64 * 3), ("pixel_size", C.c_double), ("lambda_", C.c_double),
+                ("boundary_widths", C.c_int64 * 3), ("epsilon", C.c_double), ("epsilonmin", C.c_double),
+                ("acc", C.c_int * 3), ("vector", C.c_int), ("has_roi", C.c_int), ("roi", (C.c_int64 * 4) * 2)]
+
+
+class problem_info_t(C.Structure):
+    _fields_ = [("N", C.c_int64 * 4), ("roi", (C.c_int64 * 4) * 2), ("Bl", C.c_int64 * 3), ("Br", C.c_int64 * 3),
+                ("nwiggles", C.c_int), ("epsilon", C.c_float), ("k02e", cfloat), ("k0c", C.c_double),
+                ("iterations_per_cycle", C.c_double)]
+
+
 CALLBACK = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_int64, C.POINTER(C.c_double), C.c_int64,
                        C.c_double, C.c_void_p)
 
@@ -75,6 +91,9 @@ SYMBOLS = {
     "wsim_group_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),
     "wsim_group_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int64]),
     "wsim_group_destroy": (C.c_int, [C.c_void_p]),
+    "wsim_problem_query": (C.c_int, [C.POINTER(problem_desc_t), C.POINTER(problem_info_t)]),
+    "wsim_problem_build": (C.c_int, [C.POINTER(problem_desc_t), C.c_void_p, C.POINTER(problem_info_t),
+                                     C.c_void_p, C.POINTER(wiggle_t)]),
     "wsim_last_error": (C.c_char_p, []),
     "wsim_abi_version": (C.c_int, []),
     "wsim_selftest": (C.c_int, []),

@@ -28,6 +28,10 @@ namespace wsim {
 
 static thread_local std::string g_err;
 
+// the message wsim_last_error() returns, for the entry points of other
+// translation units of the library
+void set_last_error(const std::string& msg) { g_err = msg; }
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
