@@ -242,6 +242,9 @@ int wsim_problem_build(const wsim_problem_desc* desc, const wsim_cdouble* n, wsi
     const Layout L = layout(desc);
     *info = wsim_problem_info{};
     fill_info(L, desc, info);
+    for (int w = 0; w < L.nw; ++w)
+      check(wiggles[w].pxe && wiggles[w].pye && wiggles[w].pze && wiggles[w].gx && wiggles[w].gy && wiggles[w].gz,
+            "null wiggle table");
     const double lambda = desc->lambda > 0 ? desc->lambda : 1.0;
     const double dx = desc->pixel_size;
     check(dx > 0, "pixel_size must be > 0");
@@ -317,7 +320,6 @@ int wsim_problem_build(const wsim_problem_desc* desc, const wsim_cdouble* n, wsi
       float* pe[3] = {o.pye, o.pxe, o.pze};
       wsim_cfloat* g[3] = {o.gy, o.gx, o.gz};
       for (int a = 0; a < 3; ++a) {
-        check(pe[a] != nullptr && g[a] != nullptr, "null wiggle table");
         const double s = table[(size_t)w][(size_t)a];
         const int64_t N = L.N[a];
         for (int64_t i = 0; i < N; ++i) {
