@@ -152,3 +152,24 @@ def test_warmstart_forward_graph_replays_match_eager(gpu):
         got = ws.forward(X, graph=True).clone()
         torch.cuda.synchronize()
         assert torch.equal(got, ref), k
+
+
+def test_learned_warm_start_gains_over_its_prior_and_zero_start(gpu):
+    """What the trained network gains over the method without it, on held-out
+    media (seeds 0-2; the training set used seeds 1000-1599): every start runs
+    to the zero start's absolute residual.  The straight-ray prior alone saves
+    iterations over the zero start, the network saves more on top of it, and
+    every warm start ends as close to a tight solve as the zero start does
+    (within a factor of three)."""
+    import numpy as np
+    from wavesim_amd.learned.warmstart import DEFAULT_WEIGHTS, WarmStart, evaluate
+    ws = WarmStart.load(DEFAULT_WEIGHTS)
+    rows, _ = evaluate(ws, range(3))
+    for r in rows:
+        print({k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}, flush=True)
+    its = {k: sum(r[f"{k}_its"] for r in rows) for k in ("zero", "prior", "learned")}
+    print(its, flush=True)
+    assert its["learned"] < its["prior"] < its["zero"]
+    zero_err = np.mean([r["zero_err"] for r in rows])
+    for k in ("prior", "learned"):
+        assert np.mean([r[f"{k}_err"] for r in rows]) <= 3 * zero_err
