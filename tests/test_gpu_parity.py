@@ -80,6 +80,31 @@ def test_solve_scalar(gpu, builder):
     _check_solve(Eo, so, Eg, sg)
 
 
+def _absorbing_3d_periodic_z():
+    n = cases.smooth_disorder((18, 20, 16), 5, absorption=0.02)
+    return n, cases.base_opt([4, 6, 0], 1e-6, **{"lambda": 0.8}), [(1.0, [9, 10, 8])], False
+
+
+def _vector_2d_y_polarised():
+    n = cases.smooth_disorder((24, 28), 6)
+    return n, cases.base_opt([6, 6], 1e-6), [(np.ones((24, 1, 1, 1)), [1, 1, 1, 2])], True
+
+
+def _vector_3d_z_polarised_point():
+    n = cases.sphere((16, 16, 16), 4, 1.5, 1.2)
+    return n, cases.base_opt([4, 4, 4], 1e-5), [(1.0, [8, 8, 8, 3])], True
+
+
+@pytest.mark.parametrize("builder", [_absorbing_3d_periodic_z, _vector_2d_y_polarised, _vector_3d_z_polarised_point])
+def test_solve_more_configurations(gpu, builder):
+    """Against the oracle: a 3-D absorbing medium with a periodic z axis
+    (Nw = 4 in 3-D) and wavelength 0.8, a 2-D vector field with a
+    y-polarised source, a 3-D vector field with a z-polarised point source."""
+    n, opt, src, vector = builder()
+    o, g, Eo, so, Eg, sg = _run_both(n, opt, src, vector=vector)
+    _check_solve(Eo, so, Eg, sg)
+
+
 def test_solve_vector(gpu):
     n, opt, src = cases.case_3d_vector_sphere()
     _, _, Eo, so, Eg, sg = _run_both(n, opt, src, vector=True)
