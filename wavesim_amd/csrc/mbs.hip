@@ -281,7 +281,7 @@ static size_t lds_header(int ncol, int n, int ntw, bool fuse = false) {
 }
 
 // contig LDS line pitch: one padding slot every 8 rows, plus one
-static int contig_pitch(int n) { return n + (n >> 3) + 1; }
+static int contig_pitch(int n) { return n + (n >> WSIM_PAD_SHIFT) + 1; }
 
 static int its_for(int C, int R, bool compact = false) { return its_per_stage(ept_for(C, compact ? 1 : 0), C, R); }
 

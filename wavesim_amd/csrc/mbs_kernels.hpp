@@ -196,6 +196,10 @@ __device__ __forceinline__ float ldt(const float* p) {
 }
 __device__ __forceinline__ float2 ldt(const float2* p) { return ldg(p, 0); }
 
+#ifndef WSIM_PAD_SHIFT  // contiguous lines in LDS: one padding slot every 2^WSIM_PAD_SHIFT rows
+#define WSIM_PAD_SHIFT 3
+#endif
+
 #ifndef WSIM_EPI_CHUNK  // rows of one column whose mix operands are loaded together
 #define WSIM_EPI_CHUNK 4
 #endif
@@ -238,7 +242,7 @@ struct Pass {
   // (24-bit products: LDS indices stay far below 2^24, and a 24-bit multiply
   // issues at full rate where a 32-bit one takes four cycles)
   __device__ __forceinline__ int lidx(int t, int c, int row) const {
-    return CONTIG ? (int)__umul24((uint32_t)(c * a.ncol + t), (uint32_t)a.lds_pitch) + row + (row >> 3)
+    return CONTIG ? (int)__umul24((uint32_t)(c * a.ncol + t), (uint32_t)a.lds_pitch) + row + (row >> WSIM_PAD_SHIFT)
                   : (int)__umul24((uint32_t)row, (uint32_t)a.lds_pitch) + c * a.ncol + t;
   }
 
