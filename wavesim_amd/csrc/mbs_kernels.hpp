@@ -203,6 +203,9 @@ __device__ __forceinline__ float2 ldt(const float2* p) { return ldg(p, 0); }
 #ifndef WSIM_EPI_CHUNK  // rows of one column whose mix operands are loaded together
 #define WSIM_EPI_CHUNK 4
 #endif
+#ifndef WSIM_FUSED_CHUNK  // the same when the next forward pass is fused into the mix
+#define WSIM_FUSED_CHUNK 4
+#endif
 
 template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
 struct Pass {
@@ -493,10 +496,10 @@ struct Pass {
   template <int R>
   __device__ __forceinline__ void mix_rows(const float2* v, int it, int t, int j, int nb, const ColInfo& ci) {
     constexpr int ITS = its<R>();
-    // rows per chunk: 4 for scalar fields (2 when the next forward pass is
-    // fused, whose operand hand-off holds more registers), 1 per component
+    // rows per chunk: 4 for scalar fields (118 VGPRs in the fused variant,
+    // four waves per SIMD; 2 rows ran the pass 4% slower), 1 per component
     // for vector fields
-    constexpr int CH0 = C == 1 ? (EPI == EPI_MIX_FWD ? WSIM_EPI_CHUNK / 2 : WSIM_EPI_CHUNK) : 1;
+    constexpr int CH0 = C == 1 ? (EPI == EPI_MIX_FWD ? WSIM_FUSED_CHUNK : WSIM_EPI_CHUNK) : 1;
     constexpr int CH = CH0 < R ? CH0 : R;
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
