@@ -39,3 +39,15 @@ def test_measured_traffic_matches_algorithmic_bytes():
     # iterations (+8 B/voxel on average), the source in iterations 1-7 (Nw = 8; +8 B x 7/24)
     assert abs(sum(t["pass_inv_mix_fwd"]) / ((48 + 8 * 7 / 24) * nvox) - 1) < 0.01
     assert b.load_traffic("disordered_3d_256", "another-plan") == {}
+
+
+def test_profiles_measured_on_the_shipped_plan():
+    """bench.py reports the counter traffic only for the plan it was measured
+    with, so the committed measurement must use the plan that ships for the
+    320^3 workload."""
+    import json
+    shipped = json.load(open(os.path.join(ROOT, "wavesim_amd", "tuned_plans.json")))
+    plan = shipped["320x320x320x1|rhs1|nw8"]["plan"]
+    assert _bench().load_traffic("disordered_3d_256", plan)
+    head = open(os.path.join(ROOT, "profiles", "kernel_stats_disordered_3d_256.txt")).read(2000)
+    assert plan in head
