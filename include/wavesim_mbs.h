@@ -77,7 +77,8 @@ typedef int (*wsim_callback)(const char* type, int64_t it, const double* diff_en
  *            past it are not written); *n_diff_energy receives the number of
  *            iterations run.  The cap max_iter applies at the first multiple of
  *            Nwiggles at or after it (Simulation.m:202), so up to
- *            ceil(max_iter / nwiggles) * nwiggles iterations run.
+ *            ceil(max_iter / nwiggles) * nwiggles iterations run (at most 2 000 000,
+ *            rounded up likewise).
  * The loop follows WaveSimBase.m:157-197 exactly: source i/eps/Nw added during
  * the first Nw iterations, alpha = 0.7, energy refreshed at iterations
  * 1, 2, 2+interval, ... with interval = min(8, callback_interval)
@@ -113,7 +114,9 @@ int wsim_solver_create(wsim_solver** out, const wsim_solver_desc* desc, const ws
 int wsim_solver_set_source(wsim_solver* s, int rhs, int npatches, const wsim_source_patch* patches);
 /* Run the loop from E' = dE' = 0 for every rhs with a nonzero source.
  * batch = iterations per launch batch (0 = automatic); the batch is captured
- * once into a HIP graph and replayed. */
+ * once into a HIP graph and replayed.  max_iter >= 1; one solve runs at most
+ * 2 000 000 iterations (rounded up to a multiple of Nwiggles), the length of
+ * the device-side energy history. */
 int wsim_solver_solve(wsim_solver* s, int64_t max_iter, double energy_threshold, int64_t batch,
                       const char* callback_type, double callback_interval, wsim_callback cb,
                       void* user);

@@ -105,3 +105,16 @@ def test_group_with_warm_start_equals_single(gpu):
     assert svs[0].result(0).iterations == it_single
     assert np.array_equal(svs[0].field(0), single)
     svs[0].clear_warm_start()
+
+
+def test_group_refuses_to_run_after_a_member_was_closed(gpu):
+    import wavesim_amd as w
+    _, probs = _problems()
+    a, b = probs[0][0].solver(1), probs[1][0].solver(1)
+    g = w.SolverGroup([a, b])
+    b.close()
+    with pytest.raises(RuntimeError, match="closed"):
+        g.solve(8, 1e-6)
+    g.close()
+    with pytest.raises(RuntimeError, match="group is closed"):
+        g.solve(8, 1e-6)

@@ -239,6 +239,16 @@ def test_zero_source(gpu):
     assert st.iterations == 0 and np.all(E == 0)
 
 
+def test_result_before_any_solve(gpu):
+    """A fresh handle reports no iterations, not converged and an empty history."""
+    import wavesim_amd as w
+    n, opt, _ = cases.case_2d_homogeneous()
+    sv = w.WaveSim(n, opt).solver(3)
+    for r in range(3):
+        res = sv.result(r)
+        assert res.iterations == 0 and not res.converged and res.diff_energy.size == 0
+
+
 def test_deterministic(gpu):
     import wavesim_amd as w
     n, opt, src = cases.case_2d_disordered()

@@ -938,6 +938,8 @@ struct Solver {
     dalloc(&d_niter, nrhs);
     dalloc(&d_last, nrhs);
     dalloc(&d_ref, nrhs);
+    // a result read before the first solve reports 0 iterations, not converged
+    for (int* p : {d_stopped, d_conv, d_niter}) HIPCHK(hipMemset(p, 0, nrhs * sizeof(int)));
     {
       std::vector<double> z(nrhs, 0.0);
       HIPCHK(hipMemcpy(d_ref, z.data(), nrhs * sizeof(double), hipMemcpyHostToDevice));
@@ -1710,7 +1712,9 @@ int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles
     // energy_calculation_interval = min(8, callback_interval) (Simulation.m:61, 135)
     d.energy_interval = (int)std::max(1.0, std::min(8.0, callback_interval > 0 ? callback_interval : 8.0));
     // maxIter: max_iterations or 100000 (WaveSimBase.m:127-131)
-    long long mi = (std::isfinite(max_iter) && max_iter > 0) ? (long long)max_iter : 100000;
+    // (clamped before the conversion: a double beyond the int64 range has no
+    // defined conversion, and the energy history stops at 2e6 iterations anyway)
+    long long mi = (std::isfinite(max_iter) && max_iter > 0) ? (long long)std::min(max_iter, 1e15) : 100000;
     Solver sv;
     sv.create(&d, gamma, wiggles);
     sv.set_source(0, npatches, patches);

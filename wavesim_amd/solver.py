@@ -89,7 +89,7 @@ class MBSSolver:
             cb = capi.NULL_CALLBACK
         rc = self._lib.wsim_solver_solve(self._h, int(max_iter), float(energy_threshold), int(batch),
                                          callback_type.encode(), float(callback_interval), cb, None)
-        if rc == -5:
+        if rc == capi.WSIM_ERR_ABORTED:
             return False
         capi.check(rc)
         return True
@@ -199,6 +199,10 @@ class SolverGroup:
 
     def solve(self, max_iter, energy_threshold, batch: int = 0):
         """max_iter, energy_threshold: one value for all problems or one per problem."""
+        if not getattr(self, "_h", None):
+            raise RuntimeError("the group is closed")
+        if any(not getattr(s, "_h", None) for s in self.solvers):
+            raise RuntimeError("a solver of this group was closed; close the group before its solvers")
         n = len(self.solvers)
         mi = np.broadcast_to(np.asarray(max_iter, dtype=np.int64), (n,)).copy()
         th = np.broadcast_to(np.asarray(energy_threshold, dtype=np.float64), (n,)).copy()
