@@ -138,13 +138,15 @@ int wsim_solver_propagate(wsim_solver* s, int wiggle, const wsim_cfloat* in, wsi
  * and the separate first pass runs only once per solve (not listed). */
 int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, double* bytes,
                         const char** names, int* nkernels);
-/* FFT plan of every non-trivial axis, as text "y:5,4,16/16/512;x:5,16,4/16;..."
+/* FFT plan of every non-trivial axis, as text "y:5,4,16/16/512;x:5,16,4/16/0/xcd;..."
  * (radices of the Stockham stages in forward order / workgroup columns,
- * 0 = widest that fits / optional workgroup size, 256 or 512).  Radices are
- * 2, 3, 4, 5, 7, 8, 11, 16 (16 not for 3-component fields).
+ * 0 = widest that fits / optional workgroup size, 256 or 512, 0 = default /
+ * optional "xcd": each of the 8 XCDs takes one contiguous range of the
+ * axis's tiles instead of every 8th tile).  Radices are 2, 3, 4, 5, 7, 8,
+ * 11, 16 (16 not for 3-component fields).  A plan changes speed only.
  * wsim_solver_tune times whole iterations of this problem for candidate radix
- * orders, tile widths and workgroup sizes, axis by axis, keeps the fastest
- * and reports it (the reference picks FFT-friendly sizes only,
+ * orders, tile widths, workgroup sizes and tile ranges, axis by axis, keeps
+ * the fastest and reports it (the reference picks FFT-friendly sizes only,
  * SimulationGrid.m:64-80; the per-size plan choice has no counterpart). */
 int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
 int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);

@@ -366,12 +366,31 @@ def test_plan_roundtrip_and_tuned_parity(gpu):
     assert sg.iterations == so.it - 1 and rel_l2(Eg, Eo) <= TOL
 
 
+@pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_3d_disordered])
+def test_xcd_tile_ranges_change_nothing(gpu, builder):
+    """Handing each XCD one contiguous range of tiles permutes the workgroups
+    only: fields and energy histories are bit-identical."""
+    import wavesim_amd as w
+    n, opt, src = builder()
+    g = w.WaveSim(n, opt)
+    sv = g.solver()
+    base = sv.plan
+    E1, s1 = g.exec(cases.gpu_source(src))
+    on = ";".join(item + ("/0" if item.count("/") == 1 else "") + "/xcd" for item in base.split(";"))
+    sv.plan = on
+    assert sv.plan == on and on.count("/xcd") == len(base.split(";"))
+    E2, s2 = g.exec(cases.gpu_source(src))
+    assert np.array_equal(E1, E2) and np.array_equal(s1.diff_energy, s2.diff_energy)
+    sv.plan = base
+    assert sv.plan == base
+
+
 def test_plan_rejects_bad_text(gpu):
     from wavesim_amd.capi import WsimError
     n, opt, _ = cases.case_3d_disordered()
     _, g = _pair(n, opt)
     sv = g.solver()
-    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y"):
+    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y", "y:4,7/0/256/xdc", "y:4,7/0/128"):
         with pytest.raises(WsimError):
             sv.plan = bad
 

@@ -343,6 +343,7 @@ struct Solver {
   std::vector<int> plan_rad[3];    // per-axis radix plan (empty: default)
   int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
   int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, 256 or 512 (0: default)
+  int plan_xcd[3] = {0, 0, 0};     // per-axis: each XCD takes one contiguous range of tiles
   std::vector<Launch> iter;  // one MBS iteration
   Launch pre{};                // first forward pass of iteration 1 when fused (fn == nullptr: none)
   PassArgs* d_pre = nullptr;
@@ -503,6 +504,7 @@ struct Solver {
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
+    L.args.xcd = plan_xcd[axis];
     L.args.nrhs = nrhs;
     L.args.grp_n = 1;
     L.args.grp_end = nullptr;
@@ -714,8 +716,9 @@ struct Solver {
     return best;
   }
 
-  // plan text: "y:8,8,5/16/512;x:4,16,5/16;z:4,16,5/32" (axis:radices/columns[/threads];
-  // columns 0 = widest tile that fits, threads omitted = the default for the axis)
+  // plan text: "y:8,8,5/16/512;x:4,16,5/16;z:4,16,5/32/0/xcd" (axis:radices/columns
+  // [/threads[/xcd]]; columns 0 = widest tile that fits, threads 0 or omitted = the
+  // default for the axis, xcd = each XCD takes one contiguous range of the tiles)
   std::string plan_string() const {
     static const char* nm = "yxz";
     std::string out;
@@ -727,7 +730,8 @@ struct Solver {
       const AxisPlan& p = tab[a].plan;
       for (int s2 = 0; s2 < p.nst; ++s2) out += (s2 ? "," : "") + std::to_string(p.f[s2].R);
       out += "/" + std::to_string(plan_ncol[a]);
-      if (plan_threads[a] > 0) out += "/" + std::to_string(plan_threads[a]);
+      if (plan_threads[a] > 0 || plan_xcd[a]) out += "/" + std::to_string(plan_threads[a]);
+      if (plan_xcd[a]) out += "/xcd";
     }
     return out;
   }
@@ -764,6 +768,10 @@ struct Solver {
       const int thr = slash2 == std::string::npos ? 0 : std::atoi(item.c_str() + slash2 + 1);
       require(thr == 0 || thr == 256 || thr == 512, "plan workgroup size must be 256 or 512 in '" + item + "'");
       plan_threads[a] = thr;
+      const size_t slash3 = slash2 == std::string::npos ? std::string::npos : item.find('/', slash2 + 1);
+      const std::string opt = slash3 == std::string::npos ? "" : item.substr(slash3 + 1);
+      require(opt.empty() || opt == "xcd", "unknown plan option '" + opt + "' in '" + item + "'");
+      plan_xcd[a] = opt == "xcd" ? 1 : 0;
       build_tables(a);
     }
     rebuild_launches();
@@ -804,19 +812,20 @@ struct Solver {
     return out;
   }
 
-  // coordinate search over (radix order, workgroup columns) per axis, timing
-  // whole iterations; keeps the fastest plan
+  // coordinate search over (radix order, workgroup columns, workgroup size,
+  // XCD tile ranges) per axis, timing whole iterations; keeps the fastest plan
   std::string tune(int reps) {
     double best = time_iteration(reps);
     for (int a : axes) {
       if (N[a] == 1) continue;
       const bool contig = (a == axes[0]);
-      auto try_cfg = [&](const std::vector<int>& rad, int ncol, int thr) {
+      auto try_cfg = [&](const std::vector<int>& rad, int ncol, int thr, int xcd) {
         const std::vector<int> old_r = plan_rad[a];
-        const int old_c = plan_ncol[a], old_t = plan_threads[a];
+        const int old_c = plan_ncol[a], old_t = plan_threads[a], old_x = plan_xcd[a];
         plan_rad[a] = rad;
         plan_ncol[a] = ncol;
         plan_threads[a] = thr;
+        plan_xcd[a] = xcd;
         double t = 1e30;
         try {
           build_tables(a);
@@ -831,6 +840,7 @@ struct Solver {
           plan_rad[a] = old_r;
           plan_ncol[a] = old_c;
           plan_threads[a] = old_t;
+          plan_xcd[a] = old_x;
         }
       };
       auto radices = [&] { return plan_rad[a].empty() ? default_radices(N[a], C) : plan_rad[a]; };
@@ -840,9 +850,11 @@ struct Solver {
       const std::vector<int> cand = contig ? std::vector<int>{1, 2, 4, 6, 8, 12, 16, 24, 32, 48}
                                            : std::vector<int>{1, 2, 4, 8, 16, 32};
       for (int round = 0; round < 2; ++round) {
-        for (const auto& rad : radix_candidates(N[a], 1, 48, C)) try_cfg(rad, plan_ncol[a], plan_threads[a]);
+        for (const auto& rad : radix_candidates(N[a], 1, 48, C))
+          try_cfg(rad, plan_ncol[a], plan_threads[a], plan_xcd[a]);
         for (int thr : {256, 512})
-          for (int c : cand) try_cfg(radices(), c, thr);
+          for (int c : cand) try_cfg(radices(), c, thr, plan_xcd[a]);
+        try_cfg(radices(), plan_ncol[a], plan_threads[a], 1 - plan_xcd[a]);
       }
       build_tables(a);
     }

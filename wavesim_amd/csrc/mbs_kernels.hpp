@@ -136,6 +136,7 @@ struct PassArgs {
   int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
   int reverse;         // walk the tiles from the end of memory (the part the
                        // previous pass wrote last, still in the memory-side cache)
+  int xcd;             // hand each XCD one contiguous range of tiles
 };
 
 // radix R is compiled for C-component fields when the R*C values of one
@@ -633,6 +634,12 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     ap += e;
   }
   const PassArgs& a = *ap;
+  if (a.xcd) {
+    // workgroups go to the 8 XCDs round-robin by index: give XCD x the
+    // contiguous block range [x*q + min(x, r), +q + (x < r)) (a bijection)
+    const int q = a.blocks >> 3, r = a.blocks & 7, x = bx & 7, j = bx >> 3;
+    bx = x * q + (x < r ? x : r) + j;
+  }
   if (a.reverse) bx = a.blocks - 1 - bx;
   extern __shared__ __align__(16) unsigned char smem[];
   const int rhs = blockIdx.y;
