@@ -376,11 +376,15 @@ def test_xcd_tile_ranges_change_nothing(gpu, builder):
     sv = g.solver()
     base = sv.plan
     E1, s1 = g.exec(cases.gpu_source(src))
+    # the default plan leaves the choice to the launch's workgroup count
+    assert "xcd" not in base
     on = ";".join(item + ("/0" if item.count("/") == 1 else "") + "/xcd" for item in base.split(";"))
-    sv.plan = on
-    assert sv.plan == on and on.count("/xcd") == len(base.split(";"))
-    E2, s2 = g.exec(cases.gpu_source(src))
-    assert np.array_equal(E1, E2) and np.array_equal(s1.diff_energy, s2.diff_energy)
+    off = on.replace("/xcd", "/noxcd")
+    for p in (on, off):
+        sv.plan = p
+        assert sv.plan == p
+        E2, s2 = g.exec(cases.gpu_source(src))
+        assert np.array_equal(E1, E2) and np.array_equal(s1.diff_energy, s2.diff_energy)
     sv.plan = base
     assert sv.plan == base
 
@@ -390,7 +394,7 @@ def test_plan_rejects_bad_text(gpu):
     n, opt, _ = cases.case_3d_disordered()
     _, g = _pair(n, opt)
     sv = g.solver()
-    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y", "y:4,7/0/256/xdc", "y:4,7/0/128"):
+    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y", "y:4,7/0/256/xdc", "y:4,7/0/128", "y:4,7/0/0/"):
         with pytest.raises(WsimError):
             sv.plan = bad
 

@@ -162,6 +162,7 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
 // One block: reduce the energy partials of every rhs, record diff_energy the
 // way Simulation.next does (Simulation.m:193-217) and advance the iteration.
 constexpr int kControlThreads = 1024;
+constexpr int kXcdAutoBlocks = 2048;  // automatic XCD tile ranges up to this many workgroups per rhs
 __device__ void control_one(DevState* st, const double* partials, int nblk) {
   constexpr int NW = kControlThreads / 64;
   __shared__ double red[NW];
@@ -343,7 +344,11 @@ struct Solver {
   std::vector<int> plan_rad[3];    // per-axis radix plan (empty: default)
   int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
   int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, 256 or 512 (0: default)
-  int plan_xcd[3] = {0, 0, 0};     // per-axis: each XCD takes one contiguous range of tiles
+  // per-axis: each XCD takes one contiguous range of tiles (1), every 8th
+  // tile (0), or -1: the former when a launch has at most kXcdAutoBlocks
+  // workgroups per rhs (measured: 1152^2 x pass, 72 workgroups, 34 -> 27 us;
+  // 320^3 passes, >= 3200 workgroups, 2-4% slower)
+  int plan_xcd[3] = {-1, -1, -1};
   std::vector<Launch> iter;  // one MBS iteration
   Launch pre{};                // first forward pass of iteration 1 when fused (fn == nullptr: none)
   PassArgs* d_pre = nullptr;
@@ -504,7 +509,7 @@ struct Solver {
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
-    L.args.xcd = plan_xcd[axis];
+    L.args.xcd = plan_xcd[axis] >= 0 ? plan_xcd[axis] : (blocks <= kXcdAutoBlocks ? 1 : 0);
     L.args.nrhs = nrhs;
     L.args.grp_n = 1;
     L.args.grp_end = nullptr;
@@ -717,8 +722,9 @@ struct Solver {
   }
 
   // plan text: "y:8,8,5/16/512;x:4,16,5/16;z:4,16,5/32/0/xcd" (axis:radices/columns
-  // [/threads[/xcd]]; columns 0 = widest tile that fits, threads 0 or omitted = the
-  // default for the axis, xcd = each XCD takes one contiguous range of the tiles)
+  // [/threads[/xcd|noxcd]]; columns 0 = widest tile that fits, threads 0 or omitted =
+  // the default for the axis; xcd = each XCD takes one contiguous range of the
+  // tiles, noxcd = every 8th tile, omitted = by the launch's workgroup count)
   std::string plan_string() const {
     static const char* nm = "yxz";
     std::string out;
@@ -730,8 +736,8 @@ struct Solver {
       const AxisPlan& p = tab[a].plan;
       for (int s2 = 0; s2 < p.nst; ++s2) out += (s2 ? "," : "") + std::to_string(p.f[s2].R);
       out += "/" + std::to_string(plan_ncol[a]);
-      if (plan_threads[a] > 0 || plan_xcd[a]) out += "/" + std::to_string(plan_threads[a]);
-      if (plan_xcd[a]) out += "/xcd";
+      if (plan_threads[a] > 0 || plan_xcd[a] >= 0) out += "/" + std::to_string(plan_threads[a]);
+      if (plan_xcd[a] >= 0) out += plan_xcd[a] ? "/xcd" : "/noxcd";
     }
     return out;
   }
@@ -770,8 +776,9 @@ struct Solver {
       plan_threads[a] = thr;
       const size_t slash3 = slash2 == std::string::npos ? std::string::npos : item.find('/', slash2 + 1);
       const std::string opt = slash3 == std::string::npos ? "" : item.substr(slash3 + 1);
-      require(opt.empty() || opt == "xcd", "unknown plan option '" + opt + "' in '" + item + "'");
-      plan_xcd[a] = opt == "xcd" ? 1 : 0;
+      require(slash3 == std::string::npos || opt == "xcd" || opt == "noxcd",
+              "unknown plan option '" + opt + "' in '" + item + "'");
+      plan_xcd[a] = slash3 == std::string::npos ? -1 : opt == "xcd" ? 1 : 0;
       build_tables(a);
     }
     rebuild_launches();
@@ -854,7 +861,7 @@ struct Solver {
           try_cfg(rad, plan_ncol[a], plan_threads[a], plan_xcd[a]);
         for (int thr : {256, 512})
           for (int c : cand) try_cfg(radices(), c, thr, plan_xcd[a]);
-        try_cfg(radices(), plan_ncol[a], plan_threads[a], 1 - plan_xcd[a]);
+        for (int x : {0, 1}) try_cfg(radices(), plan_ncol[a], plan_threads[a], x);
       }
       build_tables(a);
     }
