@@ -14,6 +14,9 @@ the reference loop on this host's cores, bounded sample, extrapolated).
 Counter collection (short runs for rocprofv3 --pmc):
     python bench.py --trace-iterations 24     # 24 iterations of the workload, then exit
     python bench.py --copy-calibration        # 1 GiB device-to-device copy, 3 times
+
+The drop-in entry point with host arrays (PCIe-inclusive, not the value):
+    python bench.py --dropin
 """
 from __future__ import annotations
 
@@ -146,6 +149,60 @@ def learned_ab(media: int):
     return out
 
 
+def dropin(sim, src, steps: int, workload: str):
+    """The MEX-shaped entry point wsim_run_algorithm end to end: host gamma,
+    wiggle tables and source in, host E' and diff_energy out, a device solver
+    built and freed inside every call (as RunWaveSimAlgorithmMex would be
+    called, WaveSimBase.m:145-146).  This is the PCIe-inclusive rate; the
+    benchmark's value is the HBM-resident solve."""
+    import ctypes as C
+    from wavesim_amd import capi
+    L = capi.lib()
+    N = (C.c_double * 4)(*[float(x) for x in sim.N])
+    roi = (C.c_double * 8)(*np.asarray(sim.roi, np.float64).T.ravel())
+    g = capi.c64(sim.gamma)
+    W = capi.Wiggles(sim.wiggles)
+    P = capi.Patches(src.positions, src.values)
+    max_it = sim.max_iterations()
+    nw = len(sim.wiggles)
+    cap = -(-max_it // nw) * nw
+    E = np.empty(sim.N, np.complex64, order="F")
+    de = (C.c_double * cap)()
+    n = C.c_int64()
+    walls = []
+    for k in range(steps + 1):  # the first call loads the kernels
+        t0 = time.perf_counter()
+        capi.check(L.wsim_run_algorithm(N, capi.ptr(g), W.n, W.arr, P.n, P.arr, float(sim.epsilon),
+                                        capi.cfloat(float(np.real(sim.k02e)), float(np.imag(sim.k02e))), roi,
+                                        float(max_it), float(sim.energy_threshold), b"NoCallback",
+                                        float(sim.callback_interval), capi.NULL_CALLBACK, None, capi.ptr(E),
+                                        de, cap, C.byref(n)))
+        if k:
+            walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    # the same solve on the resident handle (same shipped plan): its time,
+    # and E' against the drop-in's
+    sv = sim.solver(1)
+    sv.set_source(0, src.positions, src.values)
+    ts = []
+    for _ in range(steps + 1):
+        t0 = time.perf_counter()
+        sv.solve(max_it, sim.energy_threshold)
+        ts.append(time.perf_counter() - t0)
+    Er = sv.field(0)
+    rel = float(np.linalg.norm(E - Er) / np.linalg.norm(Er))
+    resident = {"plan": sv.plan, "solve_s": round(float(np.median(ts[1:])), 6),
+                "iterations": sv.result(0).iterations}
+    its = int(n.value)
+    nvox = int(np.prod(sim.N[:3]))
+    host_bytes = g.nbytes + E.nbytes  # gamma in, E' out (tables and source are small)
+    return {"mode": "dropin", "entry_point": "wsim_run_algorithm", "workload": workload,
+            "grid": list(sim.N), "iterations": its, "wall_s": round(wall, 6),
+            "pcie_inclusive_voxel_updates_per_s": round(sim.N[3] * nvox * its / wall, 1),
+            "host_bytes_moved": host_bytes, "resident": resident,
+            "e_primed_rel_l2_vs_resident": rel, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +218,8 @@ def main():
                     help="held-out media of the learned warm-start A/B (0: skip)")
     ap.add_argument("--trace-iterations", type=int, default=0,
                     help="run this many iterations of the workload and exit (counter collection)")
+    ap.add_argument("--dropin", action="store_true",
+                    help="time wsim_run_algorithm end to end (host arrays in and out) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the 1 GiB copy whose counter readings calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -194,6 +253,9 @@ def main():
     max_it = sim.max_iterations()
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f}s grid {sim.N} nw {len(sim.wiggles)}", file=sys.stderr, flush=True)
+    if args.dropin:
+        print(json.dumps(dropin(sim, src, args.steps, args.workload)), flush=True)
+        return
     if args.trace_iterations > 0:
         sv.solve(args.trace_iterations, 0.0)
         torch.cuda.synchronize()

@@ -97,3 +97,23 @@ def test_efficient_size_product_matches_oracle():
     from oracle import mbs
     for s in range(1, 1200):
         assert w.efficient_size(s) == mbs.efficient_size(s)
+
+
+def test_shipped_plans_become_a_compiled_table(tmp_path):
+    """Every tuned plan becomes one initialiser row of the table compiled into
+    the library; malformed entries stop the build."""
+    import json
+    from wavesim_amd import build, plans
+    table = json.load(open(plans.SHIPPED))
+    rows = [l for l in build.shipped_plans_inc().splitlines() if l.startswith("{")]
+    assert len(rows) == len(table) > 0
+    for k, v in table.items():
+        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", k)
+        assert "{" + ", ".join(m.groups()) + f', "{v["plan"]}"}},' in rows
+    bad = tmp_path / "bad.json"
+    bad.write_text(json.dumps({"320x320|rhs1|nw4": {"plan": "y:5,8,8/2"}}))
+    with pytest.raises(ValueError):
+        build.shipped_plans_inc(str(bad))
+    bad.write_text(json.dumps({"320x320x1x1|rhs1|nw4": {"plan": 'y:5,8,8/2"); x'}}))
+    with pytest.raises(ValueError):
+        build.shipped_plans_inc(str(bad))

@@ -389,6 +389,25 @@ def test_xcd_tile_ranges_change_nothing(gpu, builder):
     assert sv.plan == base
 
 
+def test_library_starts_from_its_shipped_plan(gpu):
+    """A solver created through the C ABI alone (no Python plan lookup) of a
+    shape with a tuned plan starts from it; other shapes keep the default."""
+    import json
+    import wavesim_amd as w
+    from wavesim_amd import plans
+    from wavesim_amd.solver import MBSSolver
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, _ = WORKLOADS["homogeneous_2d_256"]()
+    sim = w.WaveSim(nmap, opt)
+    shipped = json.load(open(plans.SHIPPED))[plans.key(sim.N, 1, len(sim.wiggles))]["plan"]
+    with MBSSolver(sim.N, sim.gamma, sim.wiggles, float(sim.epsilon), complex(sim.k02e), sim.roi) as sv:
+        assert sv.plan == shipped
+    n, opt, _ = cases.case_2d_disordered()
+    g = w.WaveSim(n, opt)
+    with MBSSolver(g.N, g.gamma, g.wiggles, float(g.epsilon), complex(g.k02e), g.roi) as sv:
+        assert "xcd" not in sv.plan and "/512" not in sv.plan
+
+
 def test_plan_rejects_bad_text(gpu):
     from wavesim_amd.capi import WsimError
     n, opt, _ = cases.case_3d_disordered()

@@ -4,12 +4,17 @@
 runtime ``csrc/mbs.hip`` and one object per kernel variant of
 ``csrc/variant.hip`` (in parallel), then links them.  Packed-FP32
 instructions are disabled on the device side
-(``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``).  The FFT plans tuned
+on MI355X (``tuned_plans.json``) are compiled into the library as a table
+(``_build/shipped_plans.inc``), so every entry point -- the MEX-shaped
+``wsim_run_algorithm`` included -- starts from them.
 """
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
+import re
 import subprocess
 import sys
 
@@ -50,9 +55,29 @@ def _run(cmd):
     return msg
 
 
+def shipped_plans_inc(path: str = os.path.join(HERE, "tuned_plans.json")) -> str:
+    """C++ initialiser rows {Ny, Nx, Nz, C, nrhs, Nw, "plan"} of the plans in
+    ``path`` (keys as ``plans.key``: "NyxNxxNzxC|rhsR|nwW")."""
+    rows = ["// generated by wavesim_amd/build.py from tuned_plans.json"]
+    with open(path) as f:
+        table = json.load(f)
+    for k in sorted(table):
+        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", k)
+        plan = table[k]["plan"]
+        if not m or not re.fullmatch(r"[xyz0-9:,/;a-z]+", plan):
+            raise ValueError(f"bad tuned plan entry {k!r}: {plan!r}")
+        rows.append("{" + ", ".join(m.groups()) + f', "{plan}"}},')
+    return "\n".join(rows) + "\n"
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), []),
+    inc = os.path.join(OBJ, "shipped_plans.inc")
+    text = shipped_plans_inc()
+    if not os.path.exists(inc) or open(inc).read() != text:
+        with open(inc, "w") as f:
+            f.write(text)
+    units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), ["-I", OBJ]),
              (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
         for c in COMPONENTS:

@@ -163,6 +163,19 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
 // way Simulation.next does (Simulation.m:193-217) and advance the iteration.
 constexpr int kControlThreads = 1024;
 constexpr int kXcdAutoBlocks = 2048;  // automatic XCD tile ranges up to this many workgroups per rhs
+
+// FFT plans tuned on MI355X for given problem shapes, applied when a solver of
+// that shape is created (the rows are generated from tuned_plans.json by the
+// build; a build without them uses the default plans)
+struct ShippedPlan {
+  int n0, n1, n2, c, nrhs, nw;
+  const char* plan;
+};
+static const ShippedPlan kShippedPlans[] = {
+#if __has_include("shipped_plans.inc")
+#include "shipped_plans.inc"
+#endif
+    {0, 0, 0, 0, 0, 0, nullptr}};
 __device__ void control_one(DevState* st, const double* partials, int nblk) {
   constexpr int NW = kControlThreads / 64;
   __shared__ double red[NW];
@@ -742,6 +755,17 @@ struct Solver {
     return out;
   }
 
+  void reset_plan() {
+    for (int a = 0; a < 3; ++a) {
+      plan_rad[a].clear();
+      plan_ncol[a] = 0;
+      plan_threads[a] = 0;
+      plan_xcd[a] = -1;
+      build_tables(a);
+    }
+    rebuild_launches();
+  }
+
   void set_plan(const std::string& txt) {
     size_t pos = 0;
     while (pos < txt.size()) {
@@ -968,6 +992,15 @@ struct Solver {
     HIPCHK(hipMemset(p_stopped, 0, sizeof(int)));
     dalloc(&p_dummy, 1);
     rebuild_launches();
+    for (const ShippedPlan* sp = kShippedPlans; sp->plan; ++sp)
+      if (sp->n0 == N[0] && sp->n1 == N[1] && sp->n2 == N[2] && sp->c == C && sp->nrhs == nrhs && sp->nw == nw) {
+        try {
+          set_plan(sp->plan);
+        } catch (const Error&) {
+          reset_plan();  // keep the default plan
+        }
+        break;
+      }
     // the probe state: one rhs, never stopped
     DevState ps{};
     ps.it = nw + 1;
