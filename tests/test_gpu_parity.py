@@ -389,6 +389,30 @@ def test_xcd_tile_ranges_change_nothing(gpu, builder):
     assert sv.plan == base
 
 
+def test_sparse_and_dense_source_uploads_agree(gpu):
+    """A source touching few voxels is uploaded as its entries and scattered;
+    a large one as a dense field.  Overlapping patches add in patch order from
+    zero either way: the solves are bit-identical."""
+    import wavesim_amd as w
+    n, opt, _ = cases.case_2d_disordered()
+    sim = w.WaveSim(n, opt)
+    Ny, Nx = sim.N[0], sim.N[1]
+    small_pos = [[10, 12, 1, 1], [9, 11, 1, 1], [30, 40, 1, 1]]
+    small_val = [np.array([[[[1.0 - 0.5j]]]]), np.full((3, 2, 1, 1), 0.25 + 2j), np.array([[[[-0.0 - 0.0j]]]])]
+    # a zero patch over half the grid makes the upload dense
+    big_pos = [[1, 1, 1, 1]] + small_pos
+    big_val = [np.zeros((Ny, Nx // 2 + 1, 1, 1), np.complex64)] + small_val
+    out = []
+    for pos, val in ((small_pos, small_val), (big_pos, big_val)):
+        sv = sim.solver(1)
+        sv.set_source(0, pos, val)
+        assert sv.solve(200, 1e-8)
+        out.append((sv.field(0), sv.result(0)))
+    (E1, r1), (E2, r2) = out
+    assert r1.iterations == r2.iterations > 0
+    assert np.array_equal(E1, E2) and np.array_equal(r1.diff_energy, r2.diff_energy)
+
+
 def test_library_starts_from_its_shipped_plan(gpu):
     """A solver created through the C ABI alone (no Python plan lookup) of a
     shape with a tuned plan starts from it; other shapes keep the default."""

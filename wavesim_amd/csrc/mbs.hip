@@ -257,6 +257,14 @@ __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2*
     x[i] = cmul(x[i], g[i % nvox]);
 }
 
+// sparse source upload: the entries of a source that touch few voxels,
+// unique indices, written into a field that is zero elsewhere
+__global__ void __launch_bounds__(256) scatter_kernel(float2* S, const unsigned long long* idx, const float2* val,
+                                                     long long m) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256)
+    S[idx[i]] = val[i];
+}
+
 // The mix epilogue adds the increments to E' two at a time, on even
 // iterations; a rhs whose last iteration was odd still lacks its last dE'.
 // One block row per rhs; blocks of a rhs that needs nothing return at once.
@@ -1014,12 +1022,17 @@ struct Solver {
   void set_source(int rhs, int np, const wsim_source_patch* patches) {
     require(rhs >= 0 && rhs < nrhs, "rhs out of range");
     require(np >= 0 && (np == 0 || patches != nullptr), "null patches");
-    // dense source, pre-multiplied by A = i/eps/Nw in single precision
-    // (WaveSimBase.m:171-172; Source.m:163-169)
+    // source pre-multiplied by A = i/eps/Nw in single precision, the patches
+    // added in order into a zero field (WaveSimBase.m:171-172; Source.m:163-169)
     const size_t fe = (size_t)field_elems();
-    std::vector<float2> h(fe, make_float2(0.f, 0.f));
     const float ar = 0.f, ai = (1.0f / epsilon) / (float)nw;  // (1i / eps) / Nw
     bool any = false;
+    struct Entry {
+      size_t g;
+      float2 v;
+    };
+    std::vector<Entry> ent;
+    size_t total = 0;
     for (int p = 0; p < np; ++p) {
       const wsim_source_patch& sp = patches[p];
       long long pos[4], sz[4];
@@ -1030,6 +1043,20 @@ struct Solver {
         require(sz[c] >= 0 && pos[c] >= 0 && pos[c] + sz[c] <= lim, "source patch outside the grid");
       }
       require(sp.values != nullptr || sz[0] * sz[1] * sz[2] * sz[3] == 0, "null patch values");
+      total += (size_t)(sz[0] * sz[1] * sz[2] * sz[3]);
+    }
+    // few voxels: upload only those (a point source at 320^3 would otherwise
+    // move a 262 MB dense field through host memory and PCIe)
+    const bool sparse = total * 4 < fe;
+    std::vector<float2> h(sparse ? 0 : fe, make_float2(0.f, 0.f));
+    if (sparse) ent.reserve(total);
+    for (int p = 0; p < np; ++p) {
+      const wsim_source_patch& sp = patches[p];
+      long long pos[4], sz[4];
+      for (int c = 0; c < 4; ++c) {
+        pos[c] = (long long)std::llround(sp.pos[c]) - 1;
+        sz[c] = sp.size[c];
+      }
       size_t k = 0;
       for (long long q = 0; q < sz[3]; ++q)
         for (long long z = 0; z < sz[2]; ++z)
@@ -1041,12 +1068,54 @@ struct Solver {
               // A * val with A = (ar + i ai)
               const float re = ar * v.re - ai * v.im;
               const float im = ar * v.im + ai * v.re;
-              h[g].x += re;
-              h[g].y += im;
+              if (sparse) {
+                ent.push_back({g, make_float2(re, im)});
+              } else {
+                h[g].x += re;
+                h[g].y += im;
+              }
               if (v.re != 0.f || v.im != 0.f) any = true;
             }
     }
-    HIPCHK(hipMemcpy(S + (size_t)rhs * fe, h.data(), fe * sizeof(float2), hipMemcpyHostToDevice));
+    float2* Sr = S + (size_t)rhs * fe;
+    if (!sparse) {
+      HIPCHK(hipMemcpy(Sr, h.data(), fe * sizeof(float2), hipMemcpyHostToDevice));
+    } else {
+      // overlapping patches add in patch order, from zero, as in the dense field
+      std::stable_sort(ent.begin(), ent.end(), [](const Entry& a, const Entry& b) { return a.g < b.g; });
+      std::vector<unsigned long long> hi;
+      std::vector<float2> hv;
+      for (size_t i = 0; i < ent.size();) {
+        float2 acc = make_float2(0.f, 0.f);
+        const size_t g = ent[i].g;
+        for (; i < ent.size() && ent[i].g == g; ++i) {
+          acc.x += ent[i].v.x;
+          acc.y += ent[i].v.y;
+        }
+        hi.push_back(g);
+        hv.push_back(acc);
+      }
+      HIPCHK(hipMemset(Sr, 0, fe * sizeof(float2)));
+      if (!hi.empty()) {
+        struct Tmp {  // device staging freed on every path out
+          unsigned long long* i = nullptr;
+          float2* v = nullptr;
+          ~Tmp() {
+            if (i) (void)hipFree(i);
+            if (v) (void)hipFree(v);
+          }
+        } t;
+        dalloc(&t.i, hi.size());
+        dalloc(&t.v, hv.size());
+        HIPCHK(hipMemcpy(t.i, hi.data(), hi.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.v, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
+        const long long m = (long long)hi.size();
+        const unsigned blocks = (unsigned)std::min<long long>((m + 255) / 256, 4096);
+        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, 0, Sr, t.i, t.v, m);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipDeviceSynchronize());
+      }
+    }
     HIPCHK(hipDeviceSynchronize());
     has_source[rhs] = any ? 1 : 0;
   }
