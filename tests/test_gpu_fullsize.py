@@ -101,3 +101,24 @@ def test_full_size_propagate_matches_rocfft(gpu, full, sim):
         e = rel_l2(sim.propagate(E, w), ref)
         print(f"wiggle {w}: rel-L2 vs rocFFT {e:.2e}", flush=True)
         assert e <= 1e-5
+
+
+def test_cfg2_1024_disordered_full_size_matches_oracle(gpu):
+    """The 1024^2 disordered workload at its full size (1152^2 grid, plane-wave
+    line source) on its shipped plan -- whose x pass hands each XCD one range
+    of tiles -- against the oracle: same iteration count, rel-L2 <= 1e-5."""
+    import wavesim_amd as w
+    from oracle import mbs
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, src = WORKLOADS["disordered_2d_1024"]()
+    sim = w.WaveSim(nmap, opt)
+    assert "/xcd" in sim.solver(1).plan
+    Eg, sg = sim.exec(src)
+    s = mbs.Source()
+    for p, v in zip(src.positions, src.values):
+        s = s + mbs.Source.point(np.asarray(v), list(p))
+    Eo, so = mbs.run(mbs.make_sim(nmap, opt), s)
+    e = rel_l2(Eg, Eo)
+    print(f"1152^2: {sg.iterations} iterations, rel-L2 vs oracle {e:.2e}", flush=True)
+    assert sg.iterations == so.it - 1 == 116
+    assert e <= 1e-5
