@@ -122,3 +122,25 @@ def test_cfg2_1024_disordered_full_size_matches_oracle(gpu):
     print(f"1152^2: {sg.iterations} iterations, rel-L2 vs oracle {e:.2e}", flush=True)
     assert sg.iterations == so.it - 1 == 116
     assert e <= 1e-5
+
+
+def test_cfg5_spheres_full_size_first_48_iterations_match_oracle(gpu):
+    """The 128^3 spheres workload at its full size (192^3 grid, plane wave) on
+    its shipped plan -- XCD tile ranges on the x passes -- against the oracle
+    over the first 48 iterations (six wiggle cycles; the full 176 take the
+    oracle about a minute)."""
+    import wavesim_amd as w
+    from oracle import mbs
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, src = WORKLOADS["spheres_3d_128"]()
+    sim = w.WaveSim(nmap, opt)
+    assert "/xcd" in sim.solver(1).plan
+    Eg, sg = sim.exec(src, max_iterations=48)
+    s = mbs.Source()
+    for p, v in zip(src.positions, src.values):
+        s = s + mbs.Source.point(np.asarray(v), list(p))
+    Eo, so = mbs.run(mbs.make_sim(nmap, opt), s, max_iterations=48)
+    e = rel_l2(Eg, Eo)
+    print(f"192^3 spheres: {sg.iterations} iterations, rel-L2 vs oracle {e:.2e}", flush=True)
+    assert sg.iterations == so.it - 1 == 48
+    assert e <= 1e-5
