@@ -17,6 +17,8 @@ Counter collection (short runs for rocprofv3 --pmc):
 
 The drop-in entry point with host arrays (PCIe-inclusive, not the value):
     python bench.py --dropin
+Plain streaming kernels on 1 GiB (the bandwidth a kernel without arithmetic reaches):
+    python bench.py --stream
 """
 from __future__ import annotations
 
@@ -107,6 +109,41 @@ def copy_calibration():
         y.copy_(x)
     torch.cuda.synchronize()
     print(f"copy calibration: 3 launches of {COPY_BYTES} bytes read + written", flush=True)
+
+
+def stream_probe(reps: int = 20):
+    """Achieved HBM bandwidth of plain streaming kernels on 1 GiB buffers
+    (read + write, write only, read only), HIP-event timed: what a kernel
+    without any arithmetic or LDS reaches on this device, next to the 8 TB/s
+    peak that the roofline fractions use."""
+    import torch
+    n = COPY_BYTES // 8
+    x = torch.ones(n, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    xr, yr = torch.view_as_real(x), torch.view_as_real(y)
+    acc = torch.empty((), dtype=torch.float32, device="cuda")
+    ops = {
+        "runtime copy (hipMemcpy device to device)": (lambda: y.copy_(x), 2 * COPY_BYTES),
+        "elementwise scale y = 2x (float32 elements)": (lambda: torch.mul(xr, 2.0, out=yr), 2 * COPY_BYTES),
+        "elementwise scale y = 2x (complex64 elements)": (lambda: torch.mul(x, 2.0, out=y), 2 * COPY_BYTES),
+        "in-place add x += 0": (lambda: xr.add_(0.0), 2 * COPY_BYTES),
+        "fill y = 0": (lambda: yr.fill_(0.0), COPY_BYTES),
+        "sum reduction (read only)": (lambda: torch.sum(xr, dim=(0, 1), out=acc), COPY_BYTES),
+    }
+    out = {}
+    for name, (fn, nbytes) in ops.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        out[name] = {"us": round(ms * 1e3, 1), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    return {"mode": "stream", "bytes_per_buffer": COPY_BYTES, "reps": reps, "ops": out}
 
 
 def load_traffic(workload: str, plan: str):
@@ -218,6 +255,8 @@ def main():
                     help="held-out media of the learned warm-start A/B (0: skip)")
     ap.add_argument("--trace-iterations", type=int, default=0,
                     help="run this many iterations of the workload and exit (counter collection)")
+    ap.add_argument("--stream", action="store_true",
+                    help="measure plain streaming kernels on 1 GiB buffers and exit")
     ap.add_argument("--dropin", action="store_true",
                     help="time wsim_run_algorithm end to end (host arrays in and out) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -230,6 +269,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.stream:
+        print(json.dumps(stream_probe()), flush=True)
         return
 
     import torch
