@@ -615,14 +615,16 @@ struct Solver {
         out.erase(out.begin());
       }
     }
-    // tile order: consecutive passes walk memory in opposite directions, so
-    // each pass starts on what its predecessor wrote last (still in the
-    // memory-side cache).  Even passes but the last run backwards; with an
-    // odd pass count only the pair (last, next iteration's first) gets none.
+    // tile order: the last two passes of the iteration walk memory from its
+    // end, the others from its start.  Measured at 320^3 (fields larger than
+    // the 256 MB memory-side cache) over all 16 orders of the 4 passes: every
+    // order with the last (mix) pass backwards ran it 5-7 us faster, this one
+    // 0.2547 s against 0.2566 s for alternating directions from the first
+    // pass; grids that fit the cache showed no difference.
     // WSIM_REVERSE=<mask> overrides (bit k: pass k backwards).
     const char* rv = std::getenv("WSIM_REVERSE");
     for (size_t k = 0; k < out.size(); ++k)
-      out[k].args.reverse = rv ? (std::atoi(rv) >> k) & 1 : (k % 2 == 0 && k + 1 < out.size()) ? 1 : 0;
+      out[k].args.reverse = rv ? (std::atoi(rv) >> k) & 1 : (k + 2 >= out.size()) ? 1 : 0;
     if (probe_mode)
       for (auto& L : out) {
         L.args.st = dprobe;

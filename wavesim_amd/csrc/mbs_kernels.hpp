@@ -134,8 +134,7 @@ struct PassArgs {
   const int* grp_end;  // [grp_n] exclusive prefix sums of the entries' widths
   int blocks;          // this entry's workgroups along x (row length of partials)
   int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
-  int reverse;         // walk the tiles from the end of memory (the part the
-                       // previous pass wrote last, still in the memory-side cache)
+  int reverse;         // walk the tiles from the end of memory
   int xcd;             // hand each XCD one contiguous range of tiles
 };
 
