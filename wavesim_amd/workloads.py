@@ -98,6 +98,8 @@ def ragged_2d(count=32, boundary=32, residual=1e-3, sizes=RAGGED_SIZES):
 
 WORKLOADS = {
     "disordered_3d_256": lambda: disordered_3d(256, 32),
+    # the medium of the 64-source batch (192^3 + 32 px boundary: a 256^3 grid)
+    "disordered_3d_192": lambda: disordered_3d(192, 32),
     "disordered_2d_1024": lambda: disordered_2d(1024, 64),
     "homogeneous_2d_256": lambda: homogeneous_2d(256, 32),
     "sphere_vector_3d_128": lambda: sphere_vector_3d(128, 32),
