@@ -63,7 +63,8 @@ def cpu_baseline(nmap, opt, source, gpu_iterations: int, seconds: float, vector=
     bounded number of iterations and extrapolates to the GPU's iteration count."""
     from oracle import mbs
     import scipy.fft
-    cores = os.cpu_count() or 1
+    host = host_cpu()
+    cores = host["threads_used"]
     o = mbs.make_sim(nmap, opt, vector=vector)
     src = mbs.prepare_source(o, mbs.Source(list(source.positions), list(source.values)))
     N = o.N
@@ -93,7 +94,32 @@ def cpu_baseline(nmap, opt, source, gpu_iterations: int, seconds: float, vector=
                 sample=f"{len(times)} iterations of the numpy/scipy.fft complex64 restatement "
                        f"(median {per_it * 1e3:.1f} ms/iteration, {cores} threads) "
                        f"extrapolated to the GPU's {gpu_iterations} iterations",
-                ms_per_iteration=per_it * 1e3)
+                ms_per_iteration=per_it * 1e3, cpu_model=host["model"],
+                logical_cpus=host["logical_cpus"], threads_used=cores)
+
+
+def host_cpu():
+    """Model name of the host CPU, the logical CPUs this process may run on,
+    and the threads the CPU baseline uses: all of them, capped by
+    OMP_NUM_THREADS where it is set (the CPU share of a shared host)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        logical = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        logical = os.cpu_count() or 1
+    threads = logical
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return dict(model=model, logical_cpus=logical, threads_used=threads)
 
 
 COPY_BYTES = 1 << 30

@@ -58,8 +58,11 @@ typedef struct {
 } wsim_source_patch;
 
 /* Progress callback (the role of SimulationCallback.m, invoked by the MEX).
- * Called between batches of iterations with the diff_energy history so far.
- * Return nonzero to stop the solve (WSIM_ERR_ABORTED). */
+ * Called as Simulation.next calls its callback (Simulation.m:213): after
+ * every iteration that is a multiple of callback_interval and after the last
+ * one, with the diff_energy history so far (the iterations run in batches
+ * that end on those multiples).  Return nonzero to stop the solve
+ * (WSIM_ERR_ABORTED). */
 typedef int (*wsim_callback)(const char* type, int64_t it, const double* diff_energy,
                              int64_t n, double energy_threshold, void* user);
 
@@ -108,13 +111,21 @@ typedef struct {
   int energy_interval;  /* energy_calculation_interval (Simulation.m:61) */
 } wsim_solver_desc;
 
+/* Supported grids (every entry point): at most 2^29 voxels (Ny*Nx*Nz; one
+ * complex64 component below 4 GiB), axis lengths whose prime factors are
+ * 2, 3, 5, 7 and 11 only (the Stockham radices; the reference's
+ * SimulationGrid.efficient_size pads non-periodic axes to such lengths,
+ * SimulationGrid.m:64-80, but leaves periodic axes as given, so a periodic
+ * axis of, e.g., 13 or 101 samples is refused with WSIM_ERR_UNSUPPORTED),
+ * and vector fields (Npol 3) with at least two axes longer than 1. */
 int wsim_solver_create(wsim_solver** out, const wsim_solver_desc* desc, const wsim_cfloat* gamma,
                        const wsim_wiggle* wiggles);
 /* Replace the source of one rhs (host patches; the library scales by i/eps/Nw). */
 int wsim_solver_set_source(wsim_solver* s, int rhs, int npatches, const wsim_source_patch* patches);
 /* Run the loop from E' = dE' = 0 for every rhs with a nonzero source.
- * batch = iterations per launch batch (0 = automatic); the batch is captured
- * once into a HIP graph and replayed.  max_iter >= 1; one solve runs at most
+ * batch = iterations per launch batch (0 = automatic; with a callback and
+ * callback_interval >= 1 the batch is the largest divisor of the interval up
+ * to 256); the batch is captured once into a HIP graph and replayed.  max_iter >= 1; one solve runs at most
  * 2 000 000 iterations (rounded up to a multiple of Nwiggles), the length of
  * the device-side energy history. */
 int wsim_solver_solve(wsim_solver* s, int64_t max_iter, double energy_threshold, int64_t batch,

@@ -68,6 +68,36 @@ def test_errors_are_codes_not_crashes():
     assert L.wsim_solver_solve(None, 10, 0.1, 0, b"x", 0.0, capi.NULL_CALLBACK, None) == -1
 
 
+def test_grid_size_limit_is_checked_before_any_device_work():
+    """The passes address one component's field with a 32-bit byte offset, so
+    a grid of more than 2^29 voxels is refused (WSIM_ERR_UNSUPPORTED) by the
+    host-side check in create(), before any HIP call.  The descriptors carry
+    an ROI outside the grid, the next host-side check: a grid the size check
+    accepts fails there instead (WSIM_ERR_ARG), so no call reaches a device."""
+    import ctypes as C
+    from wavesim_amd import capi
+    L = capi.lib()
+    dummy = (C.c_float * 2)()
+    wig = (capi.wiggle_t * 1)()
+
+    def create(ny, nx, nz, C_=1):
+        d = capi.desc_t()
+        d.N[:] = [ny, nx, nz, C_]
+        d.nrhs, d.nwiggles, d.epsilon, d.energy_interval = 1, 1, 3.0, 8
+        d.roi[0][:] = [1, 1, 1, 1]
+        d.roi[1][:] = [ny + 1, nx, nz, C_]  # outside the grid
+        h = C.c_void_p()
+        rc = L.wsim_solver_create(C.byref(h), C.byref(d), C.cast(dummy, C.c_void_p), wig)
+        assert rc != 0
+        return rc, L.wsim_last_error()
+
+    for comps in (1, 3):
+        rc, msg = create(1024, 1024, 512, comps)  # 2^29 voxels: the largest accepted
+        assert rc == -1 and b"roi" in msg, (rc, msg)
+        rc, msg = create(1024, 1024, 513, comps)
+        assert rc == -3 and b"voxels" in msg, (rc, msg)
+
+
 @pytest.mark.parametrize("builder,vector", [(cases.case_2d_disordered, False),
                                             (cases.case_3d_disordered, False),
                                             (cases.case_1d, False),

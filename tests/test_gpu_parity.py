@@ -180,6 +180,33 @@ def test_max_iterations_cap(gpu):
     assert st.iterations == 20 and not st.converged
 
 
+@pytest.mark.parametrize("interval", [1, 7, 50, 300])
+def test_callback_cadence_follows_simulation_next(gpu, interval):
+    """Simulation.next calls the callback after every iteration that is a
+    multiple of callback_interval and after the last one (Simulation.m:213):
+    ceil(iterations / interval) calls, each with the history up to its
+    iteration, and the solve itself is unchanged by the callback."""
+    import math
+    import wavesim_amd as w
+    n, opt, src = cases.case_2d_disordered()
+    # callback_interval also caps the energy cadence (Simulation.m:135): the
+    # solve without a callback uses the same options
+    opt2 = dict(opt, callback_interval=interval)
+    E0, s0 = w.WaveSim(n, opt2).exec(cases.gpu_source(src))
+    calls = []
+    g2 = w.WaveSim(n, opt2)
+    g2.callback = lambda it, hist: calls.append((it, len(hist))) and False
+    E1, s1 = g2.exec(cases.gpu_source(src))
+    it = s1.iterations
+    print(f"interval {interval}: {it} iterations, {len(calls)} calls", flush=True)
+    assert it == s0.iterations and np.array_equal(E1, E0)
+    want = [k for k in range(interval, it + 1, interval)]
+    if not want or want[-1] != it:
+        want.append(it)
+    assert len(calls) == math.ceil(it / interval)
+    assert [c[0] for c in calls] == want and all(c[0] == c[1] for c in calls)
+
+
 @pytest.mark.parametrize("max_iterations", [6, 7])
 def test_last_increment_on_odd_iteration(gpu, max_iterations):
     """E' takes its increments two at a time, on even iterations; a solve that

@@ -163,6 +163,8 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
 // way Simulation.next does (Simulation.m:193-217) and advance the iteration.
 constexpr int kControlThreads = 1024;
 constexpr int kXcdAutoBlocks = 2048;  // automatic XCD tile ranges up to this many workgroups per rhs
+// largest grid: byte offsets 8 * (nvox - 1) of one component fit 32 bits
+constexpr long long kMaxVoxels = (1ll << 29);
 
 // FFT plans tuned on MI355X for given problem shapes, applied when a solver of
 // that shape is created (the rows are generated from tuned_plans.json by the
@@ -383,6 +385,15 @@ struct Solver {
   int graph_iters = 0;
   DevState host_state{};
   long long last_iters = 0;
+  // Allocation-free mode (wsim_run_algorithm_noalloc): every device buffer is
+  // carved from a caller-provided region (arena), gamma and E' are the
+  // caller's device arrays, the caller's stream is used and no graph is
+  // captured, so a solve calls no hipMalloc, hipFree, hipStreamCreate or
+  // hipGraphInstantiate.
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool external_stream = false;
+  bool ext_gamma = false, ext_E = false;
 
   ~Solver() { release(); }
 
@@ -391,25 +402,41 @@ struct Solver {
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
-    void* ptrs[] = {gamma, E, dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
+    void* ptrs[] = {dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
                     d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args, d_ref, W0, d_pre};
-    for (void* p : ptrs)
-      if (p) (void)hipFree(p);
+    for (void* p : ptrs) dfree(p);
+    if (!ext_gamma) dfree(gamma);
+    if (!ext_E) dfree(E);
     for (int a = 0; a < 3; ++a) {
-      if (d_pe[a]) (void)hipFree(d_pe[a]);
-      if (d_ramp[a]) (void)hipFree(d_ramp[a]);
-      if (d_twf[a]) (void)hipFree(d_twf[a]);
-      if (d_twi[a]) (void)hipFree(d_twi[a]);
+      dfree(d_pe[a]);
+      dfree(d_ramp[a]);
+      dfree(d_twf[a]);
+      dfree(d_twi[a]);
     }
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && !external_stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
 
   long long field_elems() const { return (long long)C * nvox; }
 
   template <class T_>
-  static void dalloc(T_** p, size_t n) {
-    HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T_)));
+  void dalloc(T_** p, size_t n) {
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T_);
+    if (arena) {
+      const size_t b = (bytes + 255) & ~(size_t)255;
+      require(arena_used + b <= arena_cap,
+              "problem needs more than the " + std::to_string(arena_cap >> 20) +
+                  " MiB scratch of the allocation-free entry point",
+              WSIM_ERR_UNSUPPORTED);
+      *p = reinterpret_cast<T_*>(arena + arena_used);
+      arena_used += b;
+      return;
+    }
+    HIPCHK(hipMalloc((void**)p, bytes));
+  }
+  // frees what dalloc allocated (arena memory is the caller's: nothing to do)
+  void dfree(void* p) {
+    if (p && !arena) (void)hipFree(p);
   }
 
   // pass configuration: columns per workgroup and LDS size
@@ -676,8 +703,8 @@ struct Solver {
     std::vector<int> rad = plan_rad[a].empty() ? default_radices(N[a], C) : plan_rad[a];
     if (N[a] == 1) rad = {};
     tab[a] = build_axis(N[a], rad);
-    if (d_twf[a]) (void)hipFree(d_twf[a]);
-    if (d_twi[a]) (void)hipFree(d_twi[a]);
+    dfree(d_twf[a]);
+    dfree(d_twi[a]);
     d_twf[a] = d_twi[a] = nullptr;
     dalloc(&d_twf[a], tab[a].twf.size());
     dalloc(&d_twi[a], tab[a].twi.size());
@@ -692,17 +719,17 @@ struct Solver {
     build_iteration(it2, false);
     const int nb = (int)it2.back().grid.x;
     if (!partials || nb > nblk) {
-      if (partials) (void)hipFree(partials);
+      dfree(partials);
       partials = nullptr;
       dalloc(&partials, (size_t)nb * nrhs);
     }
     nblk = nb;
     for (auto& L : it2) L.args.partials = partials;
-    if (d_args) (void)hipFree(d_args);
+    dfree(d_args);
     d_args = nullptr;
     iter.swap(it2);
     upload_args(iter, &d_args);
-    if (d_pre) (void)hipFree(d_pre);
+    dfree(d_pre);
     d_pre = nullptr;
     if (pre.fn) {
       std::vector<Launch> one{pre};
@@ -710,7 +737,7 @@ struct Solver {
       pre.dargs = d_pre;
     }
     probe_iter.clear();
-    if (d_probe_args) (void)hipFree(d_probe_args);
+    dfree(d_probe_args);
     d_probe_args = nullptr;
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -939,7 +966,12 @@ struct Solver {
     epsilon = d->epsilon;
     k02e = make_float2(d->k02e.re, d->k02e.im);
     nvox = (long long)N[0] * N[1] * N[2];
-    require((long long)C * nvox < (1ll << 31), "grid too large for 32-bit in-field indexing",
+    // the passes address a component's field through a 32-bit byte offset
+    // from a per-workgroup scalar base (global_load/store with a 32-bit lane
+    // offset): an element offset reaches nvox - 1 in the strided passes, so
+    // one component's field must stay below 4 GiB
+    require(nvox <= kMaxVoxels, "grid of " + std::to_string(nvox) + " voxels: at most " +
+                                    std::to_string(kMaxVoxels) + " (one component below 4 GiB)",
             WSIM_ERR_UNSUPPORTED);
     for (int c = 0; c < 4; ++c) {
       const long long lim = c < 3 ? N[c] : C;
@@ -962,7 +994,7 @@ struct Solver {
       lds_limit = std::min<size_t>((size_t)lim, env_l ? (size_t)std::atoi(env_l) * 1024 : LDS_CAP);
       lds_device = std::max<size_t>((size_t)lim, lds_limit);
     }
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!external_stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // tables
     for (int a = 0; a < 3; ++a) {
       require(N[a] == 1 || !default_radices(N[a], C).empty(),
@@ -975,9 +1007,13 @@ struct Solver {
     upload_wiggles(wig);
     // fields
     const size_t fe = (size_t)field_elems() * nrhs;
-    dalloc(&gamma, nvox);
-    HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
-    dalloc(&E, fe);
+    if (ext_gamma) {
+      gamma = reinterpret_cast<float2*>(const_cast<wsim_cfloat*>(g));  // the caller's device array
+    } else {
+      dalloc(&gamma, nvox);
+      HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    if (!ext_E) dalloc(&E, fe);
     dalloc(&dE, fe);
     dalloc(&T, fe);
     dalloc(&S, fe);
@@ -1100,13 +1136,14 @@ struct Solver {
       HIPCHK(hipMemset(Sr, 0, fe * sizeof(float2)));
       if (!hi.empty()) {
         struct Tmp {  // device staging freed on every path out
+          Solver* s;
           unsigned long long* i = nullptr;
           float2* v = nullptr;
           ~Tmp() {
-            if (i) (void)hipFree(i);
-            if (v) (void)hipFree(v);
+            s->dfree(i);
+            s->dfree(v);
           }
-        } t;
+        } t{this};
         dalloc(&t.i, hi.size());
         dalloc(&t.v, hv.size());
         HIPCHK(hipMemcpy(t.i, hi.data(), hi.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
@@ -1168,8 +1205,8 @@ struct Solver {
 
   void ensure_history(int cap) {
     if (cap <= hist_cap) return;
-    if (d_diff) (void)hipFree(d_diff);
-    if (d_eall) (void)hipFree(d_eall);
+    dfree(d_diff);
+    dfree(d_eall);
     d_diff = d_eall = nullptr;
     dalloc(&d_diff, (size_t)cap * nrhs);
     dalloc(&d_eall, (size_t)cap * nrhs);
@@ -1241,14 +1278,23 @@ struct Solver {
     require(max_iter >= 1, "max_iter must be >= 1");
     ensure_history(history_for(max_iter));
     reset_state(max_iter, thr);
-    if (batch <= 0) {
+    if (cb != nullptr && cbint >= 1.0) {
+      // the reference calls the callback at every multiple of
+      // callback_interval and at the stop (Simulation.m:213): batches end on
+      // those iterations (the largest divisor of the interval up to 256
+      // iterations, so a long interval does not make one huge graph)
+      const long long ci = std::max<long long>(1, std::llround(cbint));
+      long long d = std::min<long long>(ci, 256);
+      while (ci % d != 0) --d;
+      batch = d;
+    } else if (batch <= 0) {
       // about 2 ms of work per batch, a multiple of Nw
       const double it_us = 112.0 * (double)nvox * C * nrhs / 5e6;  // ~5 TB/s
       long long k = (long long)std::ceil(2000.0 / std::max(it_us, 1.0));
       k = std::max<long long>(nw, std::min<long long>(k, 256));
       batch = ((k + nw - 1) / nw) * nw;
     }
-    const bool use_graph = std::getenv("WSIM_NO_GRAPH") == nullptr;
+    const bool use_graph = std::getenv("WSIM_NO_GRAPH") == nullptr && arena == nullptr;
     if (use_graph) capture_graph((int)batch);
     long long done = 0;
     double next_cb = cbint > 0 ? cbint : INFINITY;
@@ -1403,6 +1449,11 @@ static int guard(F&& f) {
 // ----------------------------------------------------------------------------
 constexpr int kGroupMax = 64;
 
+template <class T_>
+static void hip_alloc(T_** p, size_t n) {
+  HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T_)));
+}
+
 struct Group {
   std::vector<Solver*> ps;
   hipStream_t stream = nullptr;
@@ -1480,8 +1531,8 @@ struct Group {
       const size_t K = ps[sb.members[0]]->iter.size(), M = sb.members.size();
       std::vector<PassArgs> tab(K * M);
       std::vector<int> ends(K * M);
-      Solver::dalloc(&sb.d_tab, K * M);
-      Solver::dalloc(&sb.d_end, K * M);
+      hip_alloc(&sb.d_tab, K * M);
+      hip_alloc(&sb.d_end, K * M);
       for (size_t k = 0; k < K; ++k) {
         Launch G = ps[sb.members[0]]->iter[k];
         long long x = 0;
@@ -1510,8 +1561,8 @@ struct Group {
       HIPCHK(hipMemcpy(sb.d_end, ends.data(), ends.size() * sizeof(int), hipMemcpyHostToDevice));
       subs.push_back(std::move(sb));
     }
-    Solver::dalloc(&d_done, ps.size());
-    Solver::dalloc(&d_ctl, ps.size());
+    hip_alloc(&d_done, ps.size());
+    hip_alloc(&d_ctl, ps.size());
     std::vector<CtlEntry> ce(ps.size());
     for (size_t i = 0; i < ps.size(); ++i)
       ce[i] = CtlEntry{ps[i]->dst, ps[i]->partials, ps[i]->nblk, d_done + i};
