@@ -94,6 +94,33 @@ int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles
                        int64_t diff_energy_cap, int64_t* n_diff_energy);
 
 /* ---------------------------------------------------------------------------
+ * Allocation-free variant of wsim_run_algorithm.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * must not call hipMalloc, such as tight serving loops or harnesses that
+ * forbid allocation).  It calls no hipMalloc, hipFree, stream or graph
+ * creation: its work fields (dE', T, S), tables, state and histories live in
+ * a static 64 MiB device array that every GPU loading the library holds
+ * until the process ends.
+ *  gamma_dev  device array, Ny*Nx*Nz complex (read only)
+ *  E_out_dev  device array, N[0]*N[1]*N[2]*N[3] complex: receives E' (full grid)
+ *  stream     hipStream_t the kernels run on (NULL: the null stream); the
+ *             call waits for the stream's earlier work first and returns
+ *             when the solve is done
+ * Other arguments and results as wsim_run_algorithm (host tables, patches,
+ * diff_energy_out).  Limits, refused with WSIM_ERR_UNSUPPORTED: 3 * Npol *
+ * Ny*Nx*Nz * 8 bytes of work fields within the 64 MiB (plus small tables;
+ * e.g. 1152^2 and 128^3 scalar fit, 320^3 does not) and at most 100 000
+ * iterations.  Calls on one device share the array: within a process they
+ * run one at a time (a lock); the array keeps the last call's contents, and
+ * every call clears what must start at zero. */
+int wsim_run_algorithm_noalloc(const double N[4], const wsim_cfloat* gamma_dev, int nwiggles,
+                               const wsim_wiggle* wiggles, int npatches, const wsim_source_patch* patches,
+                               float epsilon, wsim_cfloat k02e, const double roi[8], double max_iter,
+                               double energy_threshold, const char* callback_type, double callback_interval,
+                               wsim_callback cb, void* user, wsim_cfloat* E_out_dev, double* diff_energy_out,
+                               int64_t diff_energy_cap, int64_t* n_diff_energy, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Handle API: the same loop with device-resident buffers, for repeated solves
  * and for nrhs right-hand sides that share gamma (one launch per pass covers
  * every rhs; each rhs has its own energy history and stop flag).  There is no

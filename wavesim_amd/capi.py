@@ -69,6 +69,12 @@ SYMBOLS = {
                                      C.POINTER(C.c_double), C.c_double, C.c_double, C.c_char_p,
                                      C.c_double, CALLBACK, C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]),
+    "wsim_run_algorithm_noalloc": (C.c_int, [C.POINTER(C.c_double), C.c_void_p, C.c_int, C.POINTER(wiggle_t),
+                                             C.c_int, C.POINTER(patch_t), C.c_float, cfloat,
+                                             C.POINTER(C.c_double), C.c_double, C.c_double, C.c_char_p,
+                                             C.c_double, CALLBACK, C.c_void_p, C.c_void_p,
+                                             C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64),
+                                             C.c_void_p]),
     "wsim_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(desc_t), C.c_void_p,
                                      C.POINTER(wiggle_t)]),
     "wsim_solver_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(patch_t)]),

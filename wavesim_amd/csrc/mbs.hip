@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1627,6 +1628,31 @@ struct Group {
   }
 };
 
+// ----------------------------------------------------------------------------
+// scratch of the allocation-free entry point: one static device array per GPU
+// (each device that loads the library holds its own copy until the process
+// ends).  Host code never uses the name `scratch` as an address: its device
+// address comes from hipGetSymbolAddress, once per device (scratch_ptr).
+// ----------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr long long kNoAllocMaxIter = 100000;  // history length held in the scratch
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;  // calls on one device run one at a time
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of the scratch array on the current device (caller holds g_scratch_mu)
+static char* scratch_ptr(int dev) {
+  require(dev >= 0 && dev < kMaxDevices, "device index out of range", WSIM_ERR_HIP);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the scratch array", WSIM_ERR_HIP);
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<char*>(g_scratch_ptr[dev]);
+}
+
 }  // namespace wsim
 
 using namespace wsim;
@@ -1898,6 +1924,66 @@ int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles
     sv.result(0, &n, &conv, diff_energy_out, nullptr, diff_energy_out ? diff_energy_cap : 0);
     if (n_diff_energy) *n_diff_energy = n;
     sv.get_field(0, E_out);
+    return rc;
+  });
+}
+
+int wsim_run_algorithm_noalloc(const double N[4], const wsim_cfloat* gamma_dev, int nwiggles,
+                               const wsim_wiggle* wiggles, int npatches, const wsim_source_patch* patches,
+                               float epsilon, wsim_cfloat k02e, const double roi[8], double max_iter,
+                               double energy_threshold, const char* callback_type, double callback_interval,
+                               wsim_callback cb, void* user, wsim_cfloat* E_out_dev, double* diff_energy_out,
+                               int64_t diff_energy_cap, int64_t* n_diff_energy, void* stream) {
+  return guard([&]() -> int {
+    require(N != nullptr && roi != nullptr && gamma_dev != nullptr && E_out_dev != nullptr, "null argument");
+    wsim_solver_desc d{};
+    for (int c = 0; c < 4; ++c) d.N[c] = (int64_t)std::llround(N[c]);
+    d.nrhs = 1;
+    d.nwiggles = nwiggles;
+    d.epsilon = epsilon;
+    d.k02e = k02e;
+    for (int c = 0; c < 4; ++c) {
+      d.roi[0][c] = (int64_t)std::llround(roi[2 * c]);
+      d.roi[1][c] = (int64_t)std::llround(roi[2 * c + 1]);
+    }
+    d.energy_interval = (int)std::max(1.0, std::min(8.0, callback_interval > 0 ? callback_interval : 8.0));
+    const long long mi = (std::isfinite(max_iter) && max_iter > 0) ? (long long)std::min(max_iter, 1e15) : 100000;
+    require(mi <= kNoAllocMaxIter, "the allocation-free entry point runs at most " +
+                                       std::to_string(kNoAllocMaxIter) + " iterations", WSIM_ERR_UNSUPPORTED);
+    // the three work fields dE', T and S dominate; the arena check below is exact
+    long long nv = 1;
+    for (int c = 0; c < 3; ++c) {
+      require(d.N[c] >= 1 && d.N[c] < (1 << 30), "grid size out of range");
+      nv *= d.N[c];
+    }
+    require(d.N[3] == 1 || d.N[3] == 3, "N(4) must be 1 (scalar) or 3 (vector)");
+    const double need = 3.0 * 8.0 * (double)d.N[3] * (double)nv;
+    require(need <= (double)kScratchBytes,
+            "the allocation-free entry point holds problems of at most " + std::to_string(kScratchBytes >> 20) +
+                " MiB of work fields (3 x Npol x voxels x 8 bytes); this one needs " +
+                std::to_string((long long)(need / (1 << 20))) + " MiB",
+            WSIM_ERR_UNSUPPORTED);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    char* base = scratch_ptr(dev);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIPCHK(hipStreamSynchronize(st));  // gamma and E' may come from the caller's pending work
+    Solver sv;
+    sv.arena = base;
+    sv.arena_cap = kScratchBytes;
+    sv.stream = st;
+    sv.external_stream = true;
+    sv.ext_gamma = sv.ext_E = true;
+    sv.E = reinterpret_cast<float2*>(E_out_dev);
+    sv.create(&d, gamma_dev, wiggles);
+    sv.set_source(0, npatches, patches);
+    const int rc = sv.solve(mi, energy_threshold, 0, callback_type, callback_interval, cb, user);
+    int64_t n = 0;
+    int conv = 0;
+    sv.result(0, &n, &conv, diff_energy_out, nullptr, diff_energy_out ? diff_energy_cap : 0);
+    if (n_diff_energy) *n_diff_energy = n;
+    HIPCHK(hipStreamSynchronize(st));
     return rc;
   });
 }
