@@ -28,7 +28,7 @@ ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
 
 # (contig, mode, prologue, epilogue) -- must match WSIM_FOR_EACH_VARIANT
 VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 3), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
-            (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
+            (1, 1, 0, 4), (1, 2, 1, 4), (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
 COMPONENTS = (1, 3)
 
 

@@ -60,6 +60,11 @@ class problem_info_t(C.Structure):
                 ("iterations_per_cycle", C.c_double)]
 
 
+class pstd_desc_t(C.Structure):
+    _fields_ = [("N", C.c_int64 * 3), ("roi", (C.c_int64 * 4) * 2), ("iterations_per_cycle", C.c_double),
+                ("energy_interval", C.c_int)]
+
+
 CALLBACK = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_int64, C.POINTER(C.c_double), C.c_int64,
                        C.c_double, C.c_void_p)
 
@@ -101,6 +106,11 @@ SYMBOLS = {
     "wsim_problem_query": (C.c_int, [C.POINTER(problem_desc_t), C.POINTER(problem_info_t)]),
     "wsim_problem_build": (C.c_int, [C.POINTER(problem_desc_t), C.c_void_p, C.POINTER(problem_info_t),
                                      C.c_void_p, C.POINTER(wiggle_t)]),
+    "wsim_pstd_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(pstd_desc_t)] + [C.c_void_p] * 6),
+    "wsim_pstd_set_source": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(patch_t)]),
+    "wsim_pstd_run": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "wsim_pstd_result": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_int64]),
+    "wsim_pstd_destroy": (C.c_int, [C.c_void_p]),
     "wsim_last_error": (C.c_char_p, []),
     "wsim_abi_version": (C.c_int, []),
     "wsim_selftest": (C.c_int, []),

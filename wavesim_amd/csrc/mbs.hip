@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <complex>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -24,6 +25,7 @@
 #include "mbs_config.hpp"
 #include "mbs_kernels.hpp"
 #include "wavesim_mbs.h"
+#include "wavesim_pstd.h"
 
 namespace wsim {
 
@@ -253,6 +255,62 @@ __global__ void __launch_bounds__(kControlThreads) control_group_kernel(const Ct
   if (threadIdx.x == 0) *c.done = c.st->all_stopped;
 }
 
+// PSTD source amplitude PSTD.default_source (PSTD.m:130-137): a CW phasor
+// exp(-2 pi i N) switched on by a raised cosine over 100 optical cycles
+__host__ __device__ inline void pstd_amplitude(double ncycle, double& re, double& im) {
+  const double env = 0.5 - 0.5 * cos(M_PI * fmin(ncycle / 100.0, 1.0));
+  const double ph = -2.0 * M_PI * ncycle;
+  re = cos(ph) * env;
+  im = sin(ph) * env;
+}
+
+// One block, after each PSTD step: reduce the energy partials, record the
+// step energy relative to |A|^2 (PSTD.m:113-116) the way Simulation.next does
+// (Simulation.m:193-217) with PSTD's can_terminate = real(A) > 0.9 (PSTD.m:120),
+// then set up the next step's amplitude and expected phase advance.
+__global__ void __launch_bounds__(kControlThreads) control_pstd_kernel(DevState* st, const double* partials,
+                                                                       int nblk) {
+  constexpr int NW = kControlThreads / 64;
+  __shared__ double red[NW];
+  if (st->all_stopped) return;
+  double e = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kControlThreads) e += partials[i];
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double en = 0.0;
+  for (int k = 0; k < NW; ++k) en += red[k];
+  const int it = st->it;
+  en /= st->amp_re * st->amp_re + st->amp_im * st->amp_im;
+  st->energy_all[it - 1] = en;
+  if (st->calc_energy) st->last_energy[0] = en;
+  const double de = st->last_energy[0];
+  st->diff[it - 1] = de;
+  const bool can = st->amp_re > 0.9;
+  if (can && de / st->diff[0] < st->threshold) {
+    st->stopped[0] = 1;
+    st->converged[0] = 1;
+    st->n_iter[0] = it;
+  } else if ((can && it >= st->max_iter) || it >= st->hist_cap) {
+    st->stopped[0] = 1;
+    st->converged[0] = 0;
+    st->n_iter[0] = it;
+  }
+  st->all_stopped = st->stopped[0];
+  st->calc_energy = ((it - 1) % st->interval) == 0;
+  st->it = it + 1;
+  // the next step's amplitude (PSTD.m:94-96) and phase advance (PSTD.m:114)
+  const double pre = st->amp_re, pim = st->amp_im;
+  double re, im;
+  pstd_amplitude((double)(it + 1) / st->ipc, re, im);
+  st->amp_re = re;
+  st->amp_im = im;
+  st->amp = make_float2((float)re, (float)im);
+  const double dph = atan2(im, re) - atan2(pim, pre);
+  st->phase = make_float2((float)cos(dph), (float)sin(dph));
+}
+
 // x[c][v] *= gamma[v] for the C components of one field (warm-start seeding)
 __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2* g, long long nvox, int C) {
   const long long n = nvox * C;
@@ -393,6 +451,13 @@ struct Solver {
   // hipGraphInstantiate.
   char* arena = nullptr;
   size_t arena_cap = 0, arena_used = 0;
+  // PSTD mode (PSTD.m:79-127): E and dE hold the two time levels, S the
+  // unscaled source, coef = [c1 | c2 | c3], srcline the source lines of the
+  // first axis; pe tables hold the unscaled p ranges, ramps are 1
+  bool pstd = false;
+  float* coef = nullptr;
+  unsigned char* srcline = nullptr;
+  double pstd_ipc = 1.0;
   bool external_stream = false;
   bool ext_gamma = false, ext_E = false;
 
@@ -408,6 +473,8 @@ struct Solver {
     for (void* p : ptrs) dfree(p);
     if (!ext_gamma) dfree(gamma);
     if (!ext_E) dfree(E);
+    dfree(coef);
+    dfree(srcline);
     for (int a = 0; a < 3; ++a) {
       dfree(d_pe[a]);
       dfree(d_ramp[a]);
@@ -566,7 +633,11 @@ struct Solver {
     // algorithmic bytes per voxel, averaged over an even and an odd
     // iteration (E' is read and written on even iterations only): the mix
     // passes read T, gamma, dE' (+ E') and write dE' (+ E') (+ the next T)
-    const double perv = epi == EPI_MIX ? 32.0 * C + 8.0 : epi == EPI_MIX_FWD ? 40.0 * C + 8.0 : 16.0 * C;
+    // (the PSTD update reads T, E, E_prev and c1..c3 and writes E_next)
+    const double perv = epi == EPI_MIX ? 32.0 * C + 8.0
+                        : epi == EPI_MIX_FWD ? 40.0 * C + 8.0
+                        : epi == EPI_PSTD ? 32.0 * C + 12.0
+                                          : 16.0 * C;
     L.bytes = perv * (double)nvox * nrhs;
     if (L.lds > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -580,17 +651,61 @@ struct Solver {
   bool fuse_first_pass() const {
     const char* e = std::getenv("WSIM_FUSE");
     if (e && std::atoi(e) == 0) return false;
-    return axes.size() >= 2 && tab[axes[0]].plan.nst >= 2;
+    return !pstd && axes.size() >= 2 && tab[axes[0]].plan.nst >= 2;
   }
 
   // the launches of one iteration (in = dE', out = T, T in place, T -> E', dE')
+  // PSTD step (PSTD.m:108-111): the forward passes on the current time
+  // level, the Laplacian in the middle pass, the inverse passes, and the
+  // update in the last pass
+  void build_pstd_step(std::vector<Launch>& out) {
+    const int D = (int)axes.size();
+    if (D == 1) {
+      out.push_back(make_launch(true, MODE_MID, PRO_SRC_RAMP, EPI_PSTD, axes[0], "pstd_step_1d"));
+    } else {
+      Launch L0 = make_launch(true, MODE_FWD, PRO_SRC_RAMP, EPI_NONE, axes[0], "pstd_fwd");
+      L0.args.out = T;
+      out.push_back(L0);
+      for (int k = 1; k < D - 1; ++k) {
+        Launch L = make_launch(false, MODE_FWD, PRO_NONE, EPI_NONE, axes[k], "pass_fwd");
+        L.args.in = T;
+        L.args.out = T;
+        out.push_back(L);
+      }
+      Launch LM = make_launch(false, MODE_MID, PRO_NONE, EPI_NONE, axes[D - 1], "pstd_mid_laplacian");
+      LM.args.in = T;
+      LM.args.out = T;
+      out.push_back(LM);
+      for (int k = D - 2; k >= 1; --k) {
+        Launch L = make_launch(false, MODE_INV, PRO_NONE, EPI_NONE, axes[k], "pass_inv");
+        L.args.in = T;
+        L.args.out = T;
+        out.push_back(L);
+      }
+      Launch LE = make_launch(true, MODE_INV, PRO_NONE, EPI_PSTD, axes[0], "pstd_inv_update");
+      LE.args.in = T;
+      out.push_back(LE);
+    }
+    for (auto& L : out) {
+      L.args.pstd = 1;
+      L.args.fld[0] = E;
+      L.args.fld[1] = dE;
+      L.args.coef = coef;
+      L.args.Sp = S;
+      L.args.srcline = srcline;
+      L.args.partials = partials;
+    }
+  }
+
   void build_iteration(std::vector<Launch>& out, bool probe_mode) {
     out.clear();
     if (!probe_mode) pre = Launch{};
     const int D = (int)axes.size();
     const int epi = probe_mode ? EPI_RAMP : EPI_MIX;
     const float2* in0 = probe_mode ? probe : dE;
-    if (D == 1) {
+    if (pstd && !probe_mode) {
+      build_pstd_step(out);
+    } else if (D == 1) {
       Launch L = make_launch(true, MODE_MID, PRO_SRC_RAMP, epi, axes[0], "pass_mid_pro_epi");
       L.args.in = in0;
       L.args.S = probe_mode ? nullptr : S;
@@ -940,7 +1055,10 @@ struct Solver {
   }
 
   void launch_control() {
-    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kControlThreads), 0, stream, dst, partials, nblk);
+    if (pstd)
+      hipLaunchKernelGGL(control_pstd_kernel, dim3(1), dim3(kControlThreads), 0, stream, dst, partials, nblk);
+    else
+      hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kControlThreads), 0, stream, dst, partials, nblk);
     HIPCHK(hipGetLastError());
   }
 
@@ -952,7 +1070,7 @@ struct Solver {
   }
 
   void create(const wsim_solver_desc* d, const wsim_cfloat* g, const wsim_wiggle* wig) {
-    require(d != nullptr && g != nullptr && wig != nullptr, "null argument");
+    require(d != nullptr && (g != nullptr || pstd) && wig != nullptr, "null argument");
     for (int a = 0; a < 3; ++a) {
       require(d->N[a] >= 1 && d->N[a] < (1 << 30), "grid size out of range");
       N[a] = (int)d->N[a];
@@ -1010,6 +1128,8 @@ struct Solver {
     const size_t fe = (size_t)field_elems() * nrhs;
     if (ext_gamma) {
       gamma = reinterpret_cast<float2*>(const_cast<wsim_cfloat*>(g));  // the caller's device array
+    } else if (pstd) {
+      gamma = nullptr;  // PSTD has no gamma
     } else {
       dalloc(&gamma, nvox);
       HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
@@ -1020,6 +1140,12 @@ struct Solver {
     dalloc(&S, fe);
     HIPCHK(hipMemset(S, 0, fe * sizeof(float2)));
     has_source.assign(nrhs, 0);
+    if (pstd) {
+      dalloc(&coef, 3 * (size_t)nvox);
+      const size_t nl = (size_t)(nvox / N[axes[0]]);
+      dalloc(&srcline, nl);
+      HIPCHK(hipMemset(srcline, 0, nl));
+    }
     // state
     dalloc(&dst, 1);
     dalloc(&dprobe, 1);
@@ -1064,7 +1190,8 @@ struct Solver {
     // source pre-multiplied by A = i/eps/Nw in single precision, the patches
     // added in order into a zero field (WaveSimBase.m:171-172; Source.m:163-169)
     const size_t fe = (size_t)field_elems();
-    const float ar = 0.f, ai = (1.0f / epsilon) / (float)nw;  // (1i / eps) / Nw
+    // (PSTD: unscaled, the amplitude of each step multiplies it in the update)
+    const float ar = pstd ? 1.f : 0.f, ai = pstd ? 0.f : (1.0f / epsilon) / (float)nw;  // (1i / eps) / Nw
     bool any = false;
     struct Entry {
       size_t g;
@@ -1156,6 +1283,27 @@ struct Solver {
         HIPCHK(hipDeviceSynchronize());
       }
     }
+    if (pstd) {
+      // lines of the first axis that carry source entries (the update reads
+      // S only there)
+      const size_t nl = (size_t)(nvox / N[axes[0]]);
+      std::vector<unsigned char> fl(nl, 0);
+      for (int p = 0; p < np; ++p) {
+        const wsim_source_patch& sp = patches[p];
+        long long pos[4], sz[4];
+        for (int c = 0; c < 4; ++c) {
+          pos[c] = (long long)std::llround(sp.pos[c]) - 1;
+          sz[c] = sp.size[c];
+        }
+        for (long long z = 0; z < sz[2]; ++z)
+          for (long long x = 0; x < sz[1]; ++x)
+            for (long long y = 0; y < sz[0]; ++y) {
+              const size_t g = (size_t)((pos[2] + z) * N[1] + (pos[1] + x)) * N[0] + (size_t)(pos[0] + y);
+              fl[g / (size_t)N[axes[0]]] = 1;
+            }
+      }
+      HIPCHK(hipMemcpy(srcline, fl.data(), nl, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipDeviceSynchronize());
     has_source[rhs] = any ? 1 : 0;
   }
@@ -1201,6 +1349,13 @@ struct Solver {
   // (whose end also stops the loop) must reach that iteration
   int history_for(long long max_iter) const {
     const long long cap = std::min<long long>(max_iter, 2000000);
+    if (pstd) {
+      // PSTD applies the cap on the first step at or after it whose source
+      // amplitude has real part above 0.9: within about one optical cycle
+      // once the ramp has passed 0.9 (from about 80 cycles on)
+      const long long ramp = (long long)std::ceil(101.0 * pstd_ipc);
+      return (int)std::min<long long>(std::max(cap, ramp) + (long long)std::ceil(2.0 * pstd_ipc) + 2, 2000001);
+    }
     return (int)(((cap + nw - 1) / nw) * nw + 1);
   }
 
@@ -1235,6 +1390,18 @@ struct Solver {
     h.energy_all = d_eall;
     h.ref_energy = d_ref;
     h.warm = warm_mode ? 1 : 0;
+    if (pstd) {
+      // step 1: A_1 = default_source(1 / ipc), previous amplitude 1 (PSTD.m:83, 94-96)
+      h.pstd = 1;
+      h.ipc = pstd_ipc;
+      double re, im;
+      pstd_amplitude(1.0 / pstd_ipc, re, im);
+      h.amp_re = re;
+      h.amp_im = im;
+      h.amp = make_float2((float)re, (float)im);
+      const double dph = std::atan2(im, re);
+      h.phase = make_float2((float)std::cos(dph), (float)std::sin(dph));
+    }
     std::vector<int> st(nrhs), zero(nrhs, 0);
     int all = 1;
     for (int r = 0; r < nrhs; ++r) {
@@ -1326,7 +1493,7 @@ struct Solver {
         }
       }
     }
-    finish_e();
+    if (!pstd) finish_e();
     HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
     last_iters = host_state.it - 1;
     return rc;
@@ -1925,6 +2092,97 @@ int wsim_run_algorithm(const double N[4], const wsim_cfloat* gamma, int nwiggles
     if (n_diff_energy) *n_diff_energy = n;
     sv.get_field(0, E_out);
     return rc;
+  });
+}
+
+// ---- PSTD (PSTD.m) ----------------------------------------------------------
+struct wsim_pstd {
+  Solver s;
+};
+
+int wsim_pstd_create(wsim_pstd** out, const wsim_pstd_desc* desc, const float* c1, const float* c2,
+                     const float* c3, const float* py, const float* px, const float* pz) {
+  return guard([&]() -> int {
+    require(out != nullptr && desc != nullptr && c1 && c2 && c3 && py && px && pz, "null argument");
+    *out = nullptr;
+    require(desc->iterations_per_cycle > 0 && std::isfinite(desc->iterations_per_cycle),
+            "iterations_per_cycle must be positive");
+    wsim_solver_desc d{};
+    for (int a = 0; a < 3; ++a) d.N[a] = desc->N[a];
+    d.N[3] = 1;
+    d.nrhs = 1;
+    d.nwiggles = 1;
+    d.epsilon = 1.f;
+    for (int c = 0; c < 4; ++c) {
+      d.roi[0][c] = desc->roi[0][c];
+      d.roi[1][c] = desc->roi[1][c];
+    }
+    d.energy_interval = desc->energy_interval;
+    // the wiggle tables carry the unscaled coordinates and unit ramps
+    const int64_t nmax = std::max(d.N[0], std::max(d.N[1], d.N[2]));
+    require(nmax >= 1 && nmax < (1 << 30), "grid size out of range");
+    std::vector<wsim_cfloat> ones((size_t)nmax, wsim_cfloat{1.f, 0.f});
+    const wsim_wiggle wg{px, py, pz, ones.data(), ones.data(), ones.data()};
+    auto* h = new wsim_pstd();
+    try {
+      h->s.pstd = true;
+      h->s.pstd_ipc = desc->iterations_per_cycle;
+      h->s.create(&d, nullptr, &wg);
+      const size_t nv = (size_t)h->s.nvox;
+      for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpy(h->s.coef + k * nv, k == 0 ? c1 : k == 1 ? c2 : c3, nv * sizeof(float),
+                         hipMemcpyHostToDevice));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return WSIM_OK;
+  });
+}
+
+int wsim_pstd_set_source(wsim_pstd* p, int np, const wsim_source_patch* patches) {
+  return guard([&]() -> int {
+    require(p != nullptr, "null handle");
+    p->s.set_source(0, np, patches);
+    return WSIM_OK;
+  });
+}
+
+int wsim_pstd_run(wsim_pstd* p, double max_iterations, double thr, int64_t* n_iter, int* converged) {
+  return guard([&]() -> int {
+    require(p != nullptr, "null handle");
+    const long long mi = (std::isfinite(max_iterations) && max_iterations > 0)
+                             ? (long long)std::min(max_iterations, 1e15)
+                             : 2000000;
+    const int rc = p->s.solve(mi, thr, 0, nullptr, 0.0, nullptr, nullptr);
+    int64_t n = 0;
+    int cv = 0;
+    p->s.result(0, &n, &cv, nullptr, nullptr, 0);
+    if (n_iter) *n_iter = n;
+    if (converged) *converged = cv;
+    return rc;
+  });
+}
+
+int wsim_pstd_result(wsim_pstd* p, wsim_cfloat* E_out, double* diff_energy, int64_t cap) {
+  return guard([&]() -> int {
+    require(p != nullptr && E_out != nullptr, "null argument");
+    Solver& v = p->s;
+    int64_t n = 0;
+    int cv = 0;
+    v.result(0, &n, &cv, diff_energy, nullptr, diff_energy ? cap : 0);
+    // step k writes fld[k & 1] (E before any step: fld[0], zero)
+    const float2* F = (n & 1) ? v.dE : v.E;
+    HIPCHK(hipMemcpy(E_out, F, (size_t)v.nvox * sizeof(float2), hipMemcpyDeviceToHost));
+    return WSIM_OK;
+  });
+}
+
+int wsim_pstd_destroy(wsim_pstd* p) {
+  return guard([&]() -> int {
+    delete p;
+    return WSIM_OK;
   });
 }
 

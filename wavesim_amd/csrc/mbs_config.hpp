@@ -21,7 +21,8 @@ using KernelFn = void (*)(const PassArgs*);
 
 // the pass variants the solver launches, as numbers (contig, mode, prologue,
 // epilogue) with mode 0 fwd / 1 inv / 2 mid, prologue 1 = source + ramps,
-// epilogue 1 = mix, 2 = ramps only, 3 = mix + the next iteration's first pass
+// epilogue 1 = mix, 2 = ramps only, 3 = mix + the next iteration's first pass,
+// 4 = the PSTD time step
 #define WSIM_FOR_EACH_VARIANT(X) \
   X(1, 0, 1, 0)                  \
   X(1, 1, 0, 1)                  \
@@ -29,6 +30,8 @@ using KernelFn = void (*)(const PassArgs*);
   X(1, 1, 0, 2)                  \
   X(1, 2, 1, 1)                  \
   X(1, 2, 1, 2)                  \
+  X(1, 1, 0, 4)                  \
+  X(1, 2, 1, 4)                  \
   X(0, 0, 0, 0)                  \
   X(0, 1, 0, 0)                  \
   X(0, 2, 0, 0)

@@ -88,11 +88,18 @@ struct DevState {
   double* diff;        // [nrhs][hist_cap]  diff_energy as the reference records it
   double* energy_all;  // [nrhs][hist_cap]  |crop dE'|^2 of every iteration
   double* ref_energy;  // [nrhs] stop-test denominator; <= 0: diff_energy(1) (Simulation.m:198)
+  // PSTD time stepping (PSTD.m:79-127): the source amplitude A of the
+  // current step in double and single precision, the expected phase advance
+  // exp(i(angle A - angle A_prev)) of the step, and steps per optical cycle
+  int pstd;
+  double amp_re, amp_im;
+  float2 amp, phase;
+  double ipc;
 };
 
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
 enum { PRO_NONE = 0, PRO_SRC_RAMP = 1 };
-enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2, EPI_MIX_FWD = 3 };
+enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2, EPI_MIX_FWD = 3, EPI_PSTD = 4 };
 
 struct PassArgs {
   int N[3];  // Ny Nx Nz
@@ -136,6 +143,17 @@ struct PassArgs {
   int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
   int reverse;         // walk the tiles from the end of memory
   int xcd;             // hand each XCD one contiguous range of tiles
+  // PSTD (PSTD.m:79-127): the two time levels E, E_prev alternate between
+  // fld[0] and fld[1] (step k reads fld[(k-1)&1], writes fld[k&1]); the
+  // middle pass multiplies by the Laplacian -|p|^2 (pe = p, unscaled)
+  // instead of the Green's function; the last pass applies the update
+  // c2 E + c1 E_prev + c3 (lap E + A S) with coef = [c1 | c2 | c3] per voxel,
+  // S the unscaled source and srcline[l] != 0 for lines that carry source
+  int pstd;
+  float2* fld[2];
+  const float* coef;
+  const float2* Sp;
+  const unsigned char* srcline;
 };
 
 // radix R is compiled for C-component fields when the R*C values of one
@@ -157,6 +175,7 @@ struct ColInfo {
   float q;       // sum of squares of the other axes' coordinates
   int valid;
   int roi_ok;    // other axes inside the ROI
+  int src;       // PSTD: the line carries source entries
 };
 
 // 32-bit element offsets from a workgroup-uniform base pointer, so the
@@ -195,6 +214,9 @@ __device__ __forceinline__ float ldt(const float* p) {
   return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)p);
 }
 __device__ __forceinline__ float2 ldt(const float2* p) { return ldg(p, 0); }
+__device__ __forceinline__ float ldf(const float* base, uint32_t i) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)base + (size_t)(i * 4u));
+}
 
 #ifndef WSIM_PAD_SHIFT  // contiguous lines in LDS: one padding slot every 2^WSIM_PAD_SHIFT rows
 #define WSIM_PAD_SHIFT 3
@@ -229,6 +251,14 @@ struct Pass {
   float2* dE_b[C];
   float2* E_b[C];
   const float2* gam_b;
+  // PSTD epilogue: current and previous time level, coefficients, source
+  const float2* fcur_b[C];
+  float2* fprv_b[C];
+  const float2* Sp_b[C];
+  const float* c1_b;
+  const float* c2_b;
+  const float* c3_b;
+  float2 amp, phase;
   uint32_t nt;   // threads of the workgroup (NT is the compile-time maximum)
   ColInfo cfix;  // strided passes: the thread's column never changes
   double eacc;
@@ -470,6 +500,10 @@ struct Pass {
           mix_rows<R>(v, it, t, j, s.nb, ci);
           continue;
         }
+        if (E2 == EPI_PSTD) {
+          pstd_rows<R>(v, it, j, s.nb, ci);
+          continue;
+        }
         const uint32_t g0 = gbase(cu, ci);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -554,6 +588,56 @@ struct Pass {
     }
   }
 
+  // PSTD update of the rows j + r*nb of one column (PSTD.m:108-116):
+  // Etmp = lap E (+ A S on source lines); E_next = c2 E + c1 E_prev + c3 Etmp
+  // into E_prev's buffer; energy partials of E_next - E exp(i dphi) in the ROI.
+  // Operands of a chunk of rows are loaded before any store (as mix_rows).
+  template <int R>
+  __device__ __forceinline__ void pstd_rows(const float2* v, int it, int j, int nb, const ColInfo& ci) {
+    constexpr int ITS = its<R>();
+    constexpr int CH = 4 < R ? 4 : R;
+    const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
+#pragma unroll
+    for (int r0 = 0; r0 < R; r0 += CH) {
+      float k1[CH], k2[CH], k3[CH];
+      float2 ec[CH][C], ep[CH][C], sv[CH][C];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        if (r0 + q >= R) break;
+        const uint32_t i = gidx(ci.off, j + (r0 + q) * nb);
+        k1[q] = ldf(c1_b, i);
+        k2[q] = ldf(c2_b, i);
+        k3[q] = ldf(c3_b, i);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          ec[q][c] = ldg(fcur_b[c], i);
+          ep[q][c] = ldg(fprv_b[c], i);
+          sv[q][c] = ci.src ? ldg(Sp_b[c], i) : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        if (r0 + q >= R) break;
+        const int row = j + (r0 + q) * nb;
+        const uint32_t i = gidx(ci.off, row);
+        const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          float2 et = v[(c * ITS + it) * R + r0 + q];
+          if (ci.src) et = cadd(et, cmul(amp, sv[q][c]));
+          const float2 e = ec[q][c], em = ep[q][c];
+          const float2 en = make_float2((k2[q] * e.x + k1[q] * em.x) + k3[q] * et.x,
+                                        (k2[q] * e.y + k1[q] * em.y) + k3[q] * et.y);
+          stg(fprv_b[c], i, en);
+          if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3]) {
+            const float2 d = csub(en, cmul(e, phase));
+            eacc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
+          }
+        }
+      }
+    }
+  }
+
   // Green's function on the forward last-stage outputs (rows j + r*nb)
   template <int R>
   __device__ __forceinline__ void stage_green(float2* v, const Stage& s) const {
@@ -569,7 +653,13 @@ struct Pass {
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
           if (C == 1) {
-            v[it * R + r] = green_scalar(ci, row, v[it * R + r]);
+            if (a.pstd) {
+              // PSTD Laplacian -|p|^2 times 1/N (PSTD.m:68, 108)
+              const float pa = pe_a[row];
+              v[it * R + r] = cscale(v[it * R + r], -(ci.q + pa * pa) * a.invN);
+            } else {
+              v[it * R + r] = green_scalar(ci, row, v[it * R + r]);
+            }
           } else {
             // f_g0_vector.m:4-8; component 0 <-> x <-> array axis 1
             const float pr = pe_a[row];
@@ -716,6 +806,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
         if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
     }
     ci.valid = valid;
+    ci.src = 0;
+    if (CONTIG && a.srcline != nullptr && valid) ci.src = a.srcline[bx * a.ncol + t];
     float2 rc = make_float2(1.f, 0.f), rc2 = make_float2(1.f, 0.f);
     float q = 0.f;
     int roi_ok = 1;
@@ -751,10 +843,24 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     p.cfix = cols[threadIdx.x & (a.ncol - 1)];
   }
   const long long rbase = (long long)rhs * a.rhs_stride + vblk;
+  // PSTD: the first pass transforms the current time level
+  const float2* in_sel = (PRO == PRO_SRC_RAMP && a.pstd) ? a.fld[(it_cur - 1) & 1] : a.in;
+  if (EPI == EPI_PSTD) {
+    p.c1_b = wave_uniform(a.coef + vblk);
+    p.c2_b = wave_uniform(a.coef + a.nvox + vblk);
+    p.c3_b = wave_uniform(a.coef + 2 * a.nvox + vblk);
+    p.amp = st->amp;
+    p.phase = st->phase;
+  }
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const long long cb = rbase + (long long)c * a.nvox;
-    p.in_b[c] = wave_uniform(a.in + cb);
+    if (EPI == EPI_PSTD) {
+      p.fcur_b[c] = wave_uniform(a.fld[(it_cur - 1) & 1] + cb);
+      p.fprv_b[c] = wave_uniform(a.fld[it_cur & 1] + cb);
+      p.Sp_b[c] = wave_uniform(a.Sp + cb);
+    }
+    p.in_b[c] = wave_uniform(in_sel + cb);
     p.out_b[c] = wave_uniform(a.out + cb);
     p.S_b[c] = wave_uniform(a.S + cb);
     p.W0_b[c] = wave_uniform(a.W0 + cb);
@@ -884,7 +990,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     }
   }
 
-  if (EPI == EPI_MIX || EPI == EPI_MIX_FWD) {
+  if (EPI == EPI_MIX || EPI == EPI_MIX_FWD || EPI == EPI_PSTD) {
     // block reduction of the ROI energy (fixed order: deterministic)
     __shared__ double red[NT / 64];
     double e = p.eacc;
