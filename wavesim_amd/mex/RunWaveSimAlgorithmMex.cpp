@@ -35,8 +35,26 @@ const wsim_cfloat* complex_single(const mxArray* a, const char* what) {
   return reinterpret_cast<const wsim_cfloat*>(mxGetComplexSingles(a));
 }
 
-// forwards the library's progress callback to SimulationCallback.m (SimulationCallback.m:1)
-int forward_callback(const char* type, int64_t, const double* de, int64_t n, double thr, void*) {
+// a real double array of at least n elements (N, roi, source positions)
+const double* real_double(const mxArray* a, size_t n, const char* what) {
+  if (!a || !mxIsDouble(a) || mxIsComplex(a) || mxGetNumberOfElements(a) < n)
+    mexErrMsgIdAndTxt("wsim:arg", "%s must be a real double array of %d elements", what, (int)n);
+  return mxGetDoubles(a);
+}
+
+// a numeric scalar (any real class: mxGetScalar converts it)
+double scalar(const mxArray* a, const char* what) {
+  if (!a || !(mxIsDouble(a) || mxIsSingle(a)) || mxGetNumberOfElements(a) < 1)
+    mexErrMsgIdAndTxt("wsim:arg", "%s must be a numeric scalar", what);
+  return mxGetScalar(a);
+}
+
+// forwards the library's progress callback to SimulationCallback.m
+// (SimulationCallback.m:1, arguments callbackType, diff_energy, energy,
+// energy_threshold).  A MATLAB error in the callback is trapped (it must not
+// unwind through the library) and stops the solve; mexFunction raises it
+// after the library returned.
+int forward_callback(const char* type, int64_t, const double* de, int64_t n, double thr, void* user) {
   if (std::strcmp(type, "NoCallback") == 0) return 0;
   mxArray* args[4];
   args[0] = mxCreateString(type);
@@ -44,8 +62,12 @@ int forward_callback(const char* type, int64_t, const double* de, int64_t n, dou
   std::memcpy(mxGetDoubles(args[1]), de, (size_t)n * sizeof(double));
   args[2] = mxCreateDoubleMatrix(0, 0, mxREAL);
   args[3] = mxCreateDoubleScalar(thr);
-  mexCallMATLAB(0, nullptr, 4, args, "SimulationCallback");
+  mxArray* err = mexCallMATLABWithTrap(0, nullptr, 4, args, "SimulationCallback");
   for (auto* a : args) mxDestroyArray(a);
+  if (err != nullptr) {
+    *static_cast<bool*>(user) = true;
+    return 1;
+  }
   return 0;
 }
 
@@ -53,29 +75,49 @@ int forward_callback(const char* type, int64_t, const double* de, int64_t n, dou
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   if (nrhs != 12) mexErrMsgIdAndTxt("wsim:arg", "RunWaveSimAlgorithmMex takes 12 inputs");
-  const double* N = mxGetDoubles(prhs[0]);
+  if (nlhs > 2) mexErrMsgIdAndTxt("wsim:arg", "RunWaveSimAlgorithmMex returns at most 2 outputs");
+  const double* N = real_double(prhs[0], 4, "N");
+  for (int c = 0; c < 4; ++c)
+    if (!(N[c] >= 1 && N[c] == std::floor(N[c]))) mexErrMsgIdAndTxt("wsim:arg", "N must hold positive integers");
   const wsim_cfloat* gamma = complex_single(prhs[1], "gamma");
-  const int nw = (int)mxGetScalar(prhs[2]);
+  if (mxGetNumberOfElements(prhs[1]) != (size_t)(N[0] * N[1] * N[2]))
+    mexErrMsgIdAndTxt("wsim:arg", "gamma must have N(1)*N(2)*N(3) elements");
+  const int nw = (int)scalar(prhs[2], "Nwiggles");
+  if (nw < 1 || !mxIsCell(prhs[3]) || mxGetNumberOfElements(prhs[3]) != (size_t)nw)
+    mexErrMsgIdAndTxt("wsim:arg", "wiggles must be a cell array of Nwiggles descriptors");
   std::vector<wsim_wiggle> wig(nw);
   for (int w = 0; w < nw; ++w) {
     const mxArray* d = mxGetCell(prhs[3], w);
+    if (!d || !mxIsStruct(d)) mexErrMsgIdAndTxt("wsim:arg", "wiggles{%d} must be a struct", w + 1);
     wig[w].pxe = real_single(field(d, 0, "pxe"), "pxe");
     wig[w].pye = real_single(field(d, 0, "pye"), "pye");
     wig[w].pze = real_single(field(d, 0, "pze"), "pze");
     wig[w].gx = complex_single(field(d, 0, "gx"), "gx");
     wig[w].gy = complex_single(field(d, 0, "gy"), "gy");
     wig[w].gz = complex_single(field(d, 0, "gz"), "gz");
+    const size_t len[3] = {(size_t)N[1], (size_t)N[0], (size_t)N[2]};
+    const char* names[6] = {"pxe", "pye", "pze", "gx", "gy", "gz"};
+    for (int k = 0; k < 6; ++k)
+      if (mxGetNumberOfElements(field(d, 0, names[k])) != len[k % 3])
+        mexErrMsgIdAndTxt("wsim:arg", "wiggles{%d}.%s has the wrong length", w + 1, names[k]);
   }
   // Source object (Source.m:17-21): cells of positions (1x4 double) and values
+  // (single: data_array cast them, Simulation.m:154, 230-248)
   const mxArray* pos = mxGetProperty(prhs[4], 0, "positions");
   const mxArray* val = mxGetProperty(prhs[4], 0, "values");
+  if (!pos || !val || !mxIsCell(pos) || !mxIsCell(val) ||
+      mxGetNumberOfElements(pos) != mxGetNumberOfElements(val))
+    mexErrMsgIdAndTxt("wsim:arg", "source must be a Source object with matching positions and values cells");
   const mwSize np = mxGetNumberOfElements(pos);
   std::vector<wsim_source_patch> patches(np);
   std::vector<std::vector<wsim_cfloat>> keep(np);
   for (mwSize p = 0; p < np; ++p) {
-    const double* pp = mxGetDoubles(mxGetCell(pos, p));
+    const double* pp = real_double(mxGetCell(pos, p), 4, "source position");
     const mxArray* v = mxGetCell(val, p);
+    if (!v || !mxIsSingle(v))
+      mexErrMsgIdAndTxt("wsim:arg", "source values must be single (the solver computes in single precision)");
     const mwSize nd = mxGetNumberOfDimensions(v);
+    if (nd > 4) mexErrMsgIdAndTxt("wsim:arg", "source values have more than 4 dimensions");
     const mwSize* dims = mxGetDimensions(v);
     size_t count = 1;
     for (int c = 0; c < 4; ++c) {
@@ -93,14 +135,15 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     }
     patches[p].values = keep[p].data();
   }
-  const float eps = (float)mxGetScalar(prhs[5]);
-  const mxComplexSingle k = mxGetComplexSingles(prhs[6])[0];
-  const double* roi = mxGetDoubles(prhs[7]);
-  const double max_iter = mxGetScalar(prhs[8]);
-  const double thr = mxGetScalar(prhs[9]);
+  const float eps = (float)scalar(prhs[5], "epsilon");
+  const wsim_cfloat k = complex_single(prhs[6], "k02e")[0];
+  const double* roi = real_double(prhs[7], 8, "roi");
+  const double max_iter = scalar(prhs[8], "maxIter");
+  const double thr = scalar(prhs[9], "energy_threshold");
   char cb[64];
-  mxGetString(prhs[10], cb, sizeof(cb));
-  const double cb_int = mxGetScalar(prhs[11]);
+  if (!mxIsChar(prhs[10]) || mxGetString(prhs[10], cb, sizeof(cb)) != 0)
+    mexErrMsgIdAndTxt("wsim:arg", "callbackString must be a char array");
+  const double cb_int = scalar(prhs[11], "callback_interval");
 
   const mwSize dims[4] = {(mwSize)N[0], (mwSize)N[1], (mwSize)N[2], (mwSize)N[3]};
   plhs[0] = mxCreateNumericArray(4, dims, mxSINGLE_CLASS, mxCOMPLEX);
@@ -111,11 +154,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   const int64_t cap = (int64_t)std::min(std::ceil(mi / std::max(nw, 1)) * std::max(nw, 1), 1e7);
   std::vector<double> de((size_t)cap);
   int64_t n = 0;
+  bool cb_failed = false;
   const int rc = wsim_run_algorithm(N, gamma, nw, wig.data(), (int)np, patches.data(), eps,
-                                    wsim_cfloat{k.real, k.imag}, roi, max_iter, thr, cb, cb_int,
-                                    forward_callback, nullptr,
+                                    k, roi, max_iter, thr, cb, cb_int,
+                                    std::strcmp(cb, "NoCallback") == 0 ? nullptr : forward_callback, &cb_failed,
                                     reinterpret_cast<wsim_cfloat*>(mxGetComplexSingles(plhs[0])),
                                     de.data(), cap, &n);
+  if (cb_failed) mexErrMsgIdAndTxt("wsim:callback", "SimulationCallback failed; the solve was stopped");
   if (rc != WSIM_OK && rc != WSIM_ERR_ABORTED)
     mexErrMsgIdAndTxt("wsim:run", "wsim_run_algorithm failed (%d): %s", rc, wsim_last_error());
   n = std::min(n, cap);
