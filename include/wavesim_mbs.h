@@ -28,7 +28,9 @@ enum {
   WSIM_ERR_HIP = -2,          /* HIP runtime error (no device, launch failure) */
   WSIM_ERR_UNSUPPORTED = -3,  /* a size/shape the kernels do not handle */
   WSIM_ERR_NOMEM = -4,        /* device allocation failed */
-  WSIM_ERR_ABORTED = -5       /* the callback asked to stop */
+  WSIM_ERR_ABORTED = -5,      /* the callback asked to stop */
+  WSIM_ERR_DIVERGED = -6      /* the stop rule saw a diverging iteration (results are readable;
+                                 the rhs reports converged = -1) */
 };
 
 typedef struct {
@@ -158,8 +160,16 @@ int wsim_solver_set_source(wsim_solver* s, int rhs, int npatches, const wsim_sou
 int wsim_solver_solve(wsim_solver* s, int64_t max_iter, double energy_threshold, int64_t batch,
                       const char* callback_type, double callback_interval, wsim_callback cb,
                       void* user);
-/* Per-rhs outcome: iterations run, converged flag, diff_energy history
- * (up to cap entries) and the every-iteration ROI energy (may be null). */
+/* Per-rhs outcome: iterations run, converged flag (1 converged, 0 stopped at
+ * the cap, -1 stopped by the failure exit), diff_energy history (up to cap
+ * entries) and the every-iteration ROI energy (may be null).
+ * Failure exit (no counterpart in the reference, which would run such a
+ * solve to its cap): the control step stops a rhs whose ROI increment energy
+ * is not finite (NaN or Inf in gamma or the source, overflow), or has grown
+ * over each of the last two wiggle cycles to above 100 times the first
+ * iteration's energy (or the fixed reference energy) -- a diverging series,
+ * e.g. with epsilon forced below the convergence bound (WaveSimBase.m:63).
+ * The solve then returns WSIM_ERR_DIVERGED. */
 int wsim_solver_result(wsim_solver* s, int rhs, int64_t* n_iter, int* converged, double* diff_energy,
                        double* energy_every, int64_t cap);
 /* Copy E' (primed, full grid, N[0]*N[1]*N[2]*N[3] entries) of one rhs to the host. */

@@ -82,3 +82,21 @@ def rel_l2(a, b) -> float:
     a = np.asarray(a, np.complex128)
     b = np.asarray(b, np.complex128)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def divergence_stop(energy_every, nw, ref=0.0):
+    """The failure exit of the library's stop rule, restated over a history of
+    ROI increment energies (one per iteration): the first iteration whose
+    energy is not finite, or, at a multiple of Nw after the second cycle,
+    exceeds the energies one and two cycles earlier (which increase) and 100
+    times max(first energy, ref).  None when no iteration qualifies."""
+    e = np.asarray(energy_every, dtype=np.float64)
+    for k in range(len(e)):
+        it = k + 1
+        if not np.isfinite(e[k]):
+            return it
+        if it % nw == 0 and it > 2 * nw:
+            e1, e2 = e[k - nw], e[k - 2 * nw]
+            if e[k] > e1 and e1 > e2 and e[k] > 100.0 * max(e[0], ref):
+                return it
+    return None

@@ -534,3 +534,47 @@ def test_warm_start_device_pointer_equals_host(gpu):
         out.append(sv.field(0))
         sv.clear_warm_start()
     assert np.array_equal(out[0], out[1])
+
+
+def test_failure_exit_on_non_finite_source(gpu):
+    """A NaN in the source makes the first increment energy non-finite: the
+    solve stops there with WSIM_ERR_DIVERGED instead of running to the cap
+    (the reference would run 100000 iterations and report not converged)."""
+    import wavesim_amd as w
+    from wavesim_amd import capi
+    n, opt, _ = cases.case_2d_disordered()
+    g = w.WaveSim(n, opt)
+    with pytest.raises(capi.WsimError) as ei:
+        g.exec(w.Source(np.array([1.0, np.nan, 1.0]), [10, 12]))
+    assert ei.value.code == capi.WSIM_ERR_DIVERGED
+    r = g.solver().result(0)
+    print(f"NaN source: stopped after {r.iterations} iterations", flush=True)
+    assert r.diverged and not r.converged and r.iterations == 1
+
+
+@pytest.mark.parametrize("builder,vector", [(cases.case_2d_disordered, False), (cases.case_3d_disordered, False),
+                                            (cases.case_3d_vector_sphere, True)])
+def test_failure_exit_on_epsilon_below_the_bound(gpu, builder, vector):
+    """epsilon forced to half the convergence bound (opt.epsilon,
+    WaveSimBase.m:63): the series diverges and the solve stops with
+    WSIM_ERR_DIVERGED where the restated rule stops on the oracle's energies
+    (within one wiggle cycle), not at the cap."""
+    import wavesim_amd as w
+    from oracle import mbs
+    from wavesim_amd import capi
+    n, opt, src = builder()
+    cls = w.WaveSimVector if vector else w.WaveSim
+    eps = float(cls(n, opt).epsilon)
+    opt2 = dict(opt, epsilon=0.5 * eps)
+    o = mbs.make_sim(n, opt2, vector=vector)
+    nw = len(o.wiggles)
+    with np.errstate(all="ignore"):
+        _, so = mbs.run(o, cases.oracle_source(src), record_every=True, max_iterations=300)
+    want = cases.divergence_stop(so.energy_every, nw)
+    g = cls(n, opt2)
+    with pytest.raises(capi.WsimError) as ei:
+        g.exec(cases.gpu_source(src))
+    assert ei.value.code == capi.WSIM_ERR_DIVERGED
+    r = g.solver().result(0)
+    print(f"{builder.__name__}: stopped after {r.iterations} iterations (rule on the oracle: {want})", flush=True)
+    assert r.diverged and want is not None and abs(r.iterations - want) <= nw

@@ -15,8 +15,9 @@ LIB_PATH = os.environ.get("WSIM_LIB", os.path.join(_HERE, "libwavesim_mbs.so"))
 
 WSIM_OK = 0
 ERRORS = {-1: "WSIM_ERR_ARG", -2: "WSIM_ERR_HIP", -3: "WSIM_ERR_UNSUPPORTED", -4: "WSIM_ERR_NOMEM",
-          -5: "WSIM_ERR_ABORTED"}
+          -5: "WSIM_ERR_ABORTED", -6: "WSIM_ERR_DIVERGED"}
 WSIM_ERR_ABORTED = -5  # the callback asked to stop
+WSIM_ERR_DIVERGED = -6  # the stop rule's failure exit (non-finite or growing increment energy)
 
 
 class WsimError(RuntimeError):

@@ -217,7 +217,22 @@ __device__ void control_one(DevState* st, const double* partials, int nblk) {
       const bool can = (it % st->nw) == 0;
       const double ref = st->ref_energy[r];
       const double d1 = ref > 0.0 ? ref : st->diff[(size_t)r * st->hist_cap];
-      if (can && de / d1 < st->threshold) {
+      // failure exit (no counterpart in the reference, which runs such a
+      // solve to its cap): a non-finite increment energy, or one that grew
+      // over each of the last two wiggle cycles to above 100 times the first
+      // iteration's (or the fixed reference) energy -- a diverging series,
+      // e.g. epsilon forced below the convergence bound (WaveSimBase.m:63)
+      bool diverged = !isfinite(en);
+      if (can && it > 2 * st->nw) {
+        const double* ea = st->energy_all + (size_t)r * st->hist_cap;
+        const double e1 = ea[it - 1 - st->nw], e2 = ea[it - 1 - 2 * st->nw];
+        diverged = diverged || (en > e1 && e1 > e2 && en > 100.0 * fmax(ea[0], ref));
+      }
+      if (diverged) {
+        st->stopped[r] = 1;
+        st->converged[r] = -1;
+        st->n_iter[r] = it;
+      } else if (can && de / d1 < st->threshold) {
         st->stopped[r] = 1;
         st->converged[r] = 1;
         st->n_iter[r] = it;
@@ -288,7 +303,11 @@ __global__ void __launch_bounds__(kControlThreads) control_pstd_kernel(DevState*
   const double de = st->last_energy[0];
   st->diff[it - 1] = de;
   const bool can = st->amp_re > 0.9;
-  if (can && de / st->diff[0] < st->threshold) {
+  if (!isfinite(en)) {  // failure exit: a non-finite step energy
+    st->stopped[0] = 1;
+    st->converged[0] = -1;
+    st->n_iter[0] = it;
+  } else if (can && de / st->diff[0] < st->threshold) {
     st->stopped[0] = 1;
     st->converged[0] = 1;
     st->n_iter[0] = it;
@@ -1496,7 +1515,20 @@ struct Solver {
     if (!pstd) finish_e();
     HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
     last_iters = host_state.it - 1;
+    if (rc == WSIM_OK && any_diverged()) {
+      g_err = "the iteration diverged (non-finite or growing increment energy)";
+      rc = WSIM_ERR_DIVERGED;
+    }
     return rc;
+  }
+
+  // some rhs stopped on the failure exit of the stop rule
+  bool any_diverged() {
+    std::vector<int> cv(nrhs);
+    HIPCHK(hipMemcpy(cv.data(), d_conv, nrhs * sizeof(int), hipMemcpyDeviceToHost));
+    for (int c : cv)
+      if (c < 0) return true;
+    return false;
   }
 
   // E' of every rhs complete (finish_e_kernel), ordered on the solver's stream
@@ -1751,7 +1783,7 @@ struct Group {
   }
 
   // every problem from E' = dE' = 0 (or its warm start) to its own stop
-  void solve(const int64_t* max_iter, const double* thr, long long batch) {
+  int solve(const int64_t* max_iter, const double* thr, long long batch) {
     int nw = 1;
     double it_us = 0.0;
     for (size_t i = 0; i < ps.size(); ++i) {
@@ -1792,6 +1824,12 @@ struct Group {
       HIPCHK(hipMemcpy(&p->host_state, p->dst, sizeof(DevState), hipMemcpyDeviceToHost));
       p->last_iters = p->host_state.it - 1;
     }
+    for (Solver* p : ps)
+      if (p->any_diverged()) {
+        g_err = "a problem of the group diverged (non-finite or growing increment energy)";
+        return WSIM_ERR_DIVERGED;
+      }
+    return WSIM_OK;
   }
 };
 
@@ -2040,8 +2078,7 @@ int wsim_group_solve(wsim_group* g, const int64_t* max_iter, const double* energ
   return guard([&]() -> int {
     require(g != nullptr && max_iter != nullptr && energy_threshold != nullptr, "null argument");
     HIPCHK(hipSetDevice(g->g.ps[0]->dev));
-    g->g.solve(max_iter, energy_threshold, batch);
-    return WSIM_OK;
+    return g->g.solve(max_iter, energy_threshold, batch);
   });
 }
 

@@ -161,6 +161,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                     reinterpret_cast<wsim_cfloat*>(mxGetComplexSingles(plhs[0])),
                                     de.data(), cap, &n);
   if (cb_failed) mexErrMsgIdAndTxt("wsim:callback", "SimulationCallback failed; the solve was stopped");
+  if (rc == WSIM_ERR_DIVERGED)
+    mexErrMsgIdAndTxt("wsim:diverged", "the iteration diverged after %d iterations: %s", (int)n, wsim_last_error());
   if (rc != WSIM_OK && rc != WSIM_ERR_ABORTED)
     mexErrMsgIdAndTxt("wsim:run", "wsim_run_algorithm failed (%d): %s", rc, wsim_last_error());
   n = std::min(n, cap);

@@ -21,6 +21,7 @@ class RhsResult:
     converged: bool
     diff_energy: np.ndarray
     energy_every: np.ndarray
+    diverged: bool = False  # stopped by the failure exit of the stop rule
 
 
 class MBSSolver:
@@ -104,7 +105,7 @@ class MBSSolver:
         capi.check(self._lib.wsim_solver_result(
             self._h, int(rhs), C.byref(n), C.byref(cv),
             de.ctypes.data_as(C.POINTER(C.c_double)), ea.ctypes.data_as(C.POINTER(C.c_double)), m))
-        return RhsResult(int(n.value), bool(cv.value), de[:m], ea[:m])
+        return RhsResult(int(n.value), cv.value == 1, de[:m], ea[:m], cv.value < 0)
 
     def field(self, rhs: int = 0) -> np.ndarray:
         """E' (primed, full grid) as a [y,x,z,pol] complex64 array."""
