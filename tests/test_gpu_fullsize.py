@@ -144,3 +144,68 @@ def test_cfg5_spheres_full_size_first_48_iterations_match_oracle(gpu):
     print(f"192^3 spheres: {sg.iterations} iterations, rel-L2 vs oracle {e:.2e}", flush=True)
     assert sg.iterations == so.it - 1 == 48
     assert e <= 1e-5
+
+
+def _fullsize_fixture(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{name}.npz not generated (python -m tests.golden.make_fullsize_fixtures)")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("name,vector", [("fullsize_disordered_3d_256", False),
+                                         ("fullsize_sphere_vector_3d_128", True)])
+def test_full_size_solve_matches_oracle_fixture(gpu, name, vector):
+    """The benchmark's 256^3 disordered workload (320^3 grid) and the 128^3
+    vector sphere (192^3 x 3) solved to convergence at full size against the
+    oracle's complex64 solves of the same inputs (tests/golden/
+    make_fullsize_fixtures.py): the same iteration count, diff_energy to
+    1e-4, the ROI field sampled every k-th voxel to rel-L2 1e-5, and two
+    projections of the whole ROI field on random phases to 1e-5 of the
+    field's norm (every voxel contributes)."""
+    import wavesim_amd as w
+    from tests.golden.make_fullsize_fixtures import projections
+    from wavesim_amd.workloads import WORKLOADS
+    d = _fullsize_fixture(name)
+    nmap, opt, src = WORKLOADS[str(d["workload"])]()
+    sim = (w.WaveSimVector if vector else w.WaveSim)(nmap, opt)
+    E, st = sim.exec(src)
+    k = int(d["step"])
+    e_s = rel_l2(E[::k, ::k, ::k], d["sample"])
+    dp = np.abs(projections(E) - d["proj"]) / float(d["norm"])
+    print(f"{name}: iterations gpu {st.iterations} oracle {int(d['iterations'])}, sample rel-L2 {e_s:.2e}, "
+          f"projections {dp.max():.2e}", flush=True)
+    assert st.iterations == int(d["iterations"]) and st.converged == bool(d["converged"])
+    assert np.allclose(st.diff_energy, d["diff_energy"], rtol=1e-4)
+    assert e_s <= 1e-5
+    assert dp.max() <= 1e-5
+
+
+def test_cfg1_homogeneous_2d_against_the_analytic_solution(gpu):
+    """The 256^2 homogeneous workload (320^2 grid, 4 px per wavelength, point
+    source at the centre) against the 2-D free-space solution (i h^2/4) H0(k r)
+    of a point source on an annulus away from the source and the boundary:
+    the rel-L2 error is the discretisation's (the complex128 oracle's is the
+    same 2.7e-4), and the GPU field equals the complex64 oracle's to 1e-5."""
+    import wavesim_amd as w
+    from oracle import analytic, mbs
+    from tests import cases
+    from wavesim_amd.workloads import homogeneous_2d
+    nmap, opt, src = homogeneous_2d(256, 32)
+    sim = w.WaveSim(nmap, opt)
+    E, st = sim.exec(src)
+    n = nmap.shape[0]
+    c = n // 2
+    yy, xx = np.meshgrid(np.arange(n) - (c - 1), np.arange(n) - (c - 1), indexing="ij")
+    r = np.hypot(yy, xx) * opt["pixel_size"]
+    ref = analytic.hankel_2d(sim.k0, opt["pixel_size"], np.where(r > 0, r, 1))
+    mask = (r > 2.0) & (r < 0.45 * n * opt["pixel_size"])
+    err = rel_l2(E[..., 0, 0][mask], ref[mask])
+    o = mbs.make_sim(nmap, opt)
+    Eo, so = mbs.run(o, mbs.Source(list(src.positions), [np.asarray(v) for v in src.values]))
+    print(f"cfg1: {st.iterations} iterations, rel-L2 vs analytic {err:.2e}, vs oracle {rel_l2(E, Eo):.2e}",
+          flush=True)
+    assert st.iterations == so.it - 1
+    assert rel_l2(E, Eo) <= 1e-5
+    assert err <= 5e-4

@@ -218,6 +218,24 @@ __device__ __forceinline__ float ldf(const float* base, uint32_t i) {
   return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)base + (size_t)(i * 4u));
 }
 
+// LDS pointers carry their address space, so the index arithmetic of every
+// LDS access is 32-bit (as a generic pointer it was widened to 64 bits, with a
+// quarter-rate 64-bit multiply-add per strided row offset)
+// (complex values are held as a plain two-float vector there: float2's
+// constructors and assignments exist for generic pointers only)
+#define WSIM_LDS __attribute__((address_space(3)))
+using lf2 = WSIM_LDS v2f*;
+using lcf2 = const WSIM_LDS v2f*;
+using lcf = const WSIM_LDS float*;
+__device__ __forceinline__ float2 ldl(lcf2 p, int i) {
+  const v2f x = p[i];
+  return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ void stl(lf2 p, int i, float2 v) {
+  const v2f x = {v.x, v.y};
+  p[i] = x;
+}
+
 #ifndef WSIM_PAD_SHIFT  // contiguous lines in LDS: one padding slot every 2^WSIM_PAD_SHIFT rows
 #define WSIM_PAD_SHIFT 3
 #endif
@@ -232,18 +250,18 @@ __device__ __forceinline__ float ldf(const float* base, uint32_t i) {
 template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
 struct Pass {
   const PassArgs& a;
-  float2* lds;
+  lf2 lds;
   const ColInfo* cols;
   int tid;
   bool add_src;
   bool warm_step;  // one of the first Nw steps of a warm start
   float inv_nw;
   uint32_t lg_ncol;
-  const float2* ramp_a;  // this wiggle's ramp along the axis
-  const float2* ramp2_a; // the next iteration's (EPI_MIX_FWD)
+  lcf2 ramp_a;  // this wiggle's ramp along the axis
+  lcf2 ramp2_a; // the next iteration's (EPI_MIX_FWD)
   bool add_src2;         // the next iteration adds the source (EPI_MIX_FWD)
   bool warm_step2;       // the next iteration adds the warm-start portion
-  const float* pe_a;
+  lcf pe_a;
   const float2* in_b[C];  // workgroup bases, per component
   float2* out_b[C];
   const float2* S_b[C];
@@ -423,28 +441,35 @@ struct Pass {
           for (int c = 0; c < C; ++c)
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              v[(c * ITS + it) * R + r] = cmul(v[(c * ITS + it) * R + r], cmul(ramp_a[cu.j + r * s.nb], rc));
+              v[(c * ITS + it) * R + r] = cmul(v[(c * ITS + it) * R + r], cmul(ldl(ramp_a, cu.j + r * s.nb), rc));
         }
         step(cu, w, s);
       }
     }
   }
 
+  // A wave with at least one live butterfly reads without branching per lane:
+  // lanes past the tile (the idle lanes of a partly filled last step) read
+  // butterfly 0's operands, whose results are never stored, so the reads of
+  // a butterfly issue together.  Waves with no live butterfly (a stage with
+  // fewer butterflies than threads) skip the reads.
   template <int R, bool FIRST = false>
-  __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, const float2* tw) const {
+  __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, lcf2 tw) const {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      if (live(cu, s)) {
-        const float2* twk = tw + s.tw + cu.k;  // table [r-1][k]: lanes with consecutive k read consecutive words
+      const bool lv = live(cu, s);
+      if (__builtin_amdgcn_ballot_w64(lv) != 0) {
+        const int t = lv ? cu.t : 0, j = lv ? cu.j : 0, k = lv ? cu.k : 0;
+        lcf2 twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const float2 x = lds[lrow(cu.t, c, cu.j, s.nb, s.nbp, r)];
-            v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, twk[(r - 1) * s.Ns]);
+            const float2 x = ldl(lds, lrow(t, c, j, s.nb, s.nbp, r));
+            v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, ldl(twk, (r - 1) * s.Ns));
           }
       } else {
 #pragma unroll
@@ -475,7 +500,7 @@ struct Pass {
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[lrow(cu.t, c, base, s.Ns, s.nsp, r)] = v[(c * ITS + it) * R + r];
+          for (int r = 0; r < R; ++r) stl(lds, lrow(cu.t, c, base, s.Ns, s.nsp, r), v[(c * ITS + it) * R + r]);
       }
       step(cu, w, s);
     }
@@ -513,7 +538,7 @@ struct Pass {
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
           } else {  // EPI_RAMP: undo the ramps only (propagation probe)
-            const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
+            const float2 ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, cmul(v[(c * ITS + it) * R + r], ramp));
           }
@@ -557,7 +582,7 @@ struct Pass {
         const int r = r0 + q;
         const int row = j + r * nb;
         const uint32_t i = gidx(ci.off, row);
-        const float2 ramp = cconj(cmul(ramp_a[row], ci.rc));
+        const float2 ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
         const float2 c1 = make_float2(1.f - 0.7f * gm[q].x, -0.7f * gm[q].y);
         const float2 g2 = cmul(gm[q], gm[q]);
         const float2 c2 = make_float2(-0.7f * g2.y, 0.7f * g2.x);
@@ -581,7 +606,7 @@ struct Pass {
             float2 x = dn;
             if (warm_step2) x = cadd(x, w0[q][c]);
             if (add_src2) x = cadd(x, ldg(S_b[c], i));
-            lds[lidx(t, c, row)] = cmul(x, cmul(ramp2_a[row], ci.rc2));
+            stl(lds, lidx(t, c, row), cmul(x, cmul(ldl(ramp2_a, row), ci.rc2)));
           }
         }
       }
@@ -748,15 +773,15 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   const size_t off_twf = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
   const size_t off_twi = off_twf + (size_t)ntf * sizeof(float2);
   const size_t off_data = off_twi + (size_t)nti * sizeof(float2);
-  float2* ramp_l = reinterpret_cast<float2*>(smem + off_ramp);
-  float2* ramp2_l = reinterpret_cast<float2*>(smem + off_ramp2);
-  float* pe_l = reinterpret_cast<float*>(smem + off_pe);
-  float2* twf_l = reinterpret_cast<float2*>(smem + off_twf);
-  float2* twi_l = reinterpret_cast<float2*>(smem + off_twi);
-  float2* lds = reinterpret_cast<float2*>(smem + off_data);
+  lf2 ramp_l = (lf2)(smem + off_ramp);
+  lf2 ramp2_l = (lf2)(smem + off_ramp2);
+  WSIM_LDS float* pe_l = (WSIM_LDS float*)(smem + off_pe);
+  lf2 twf_l = (lf2)(smem + off_twf);
+  lf2 twi_l = (lf2)(smem + off_twi);
+  lf2 lds = (lf2)(smem + off_data);
   const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
-  for (int i = threadIdx.x; i < ntf; i += nthr) twf_l[i] = ldt(a.twf + i);
-  for (int i = threadIdx.x; i < nti; i += nthr) twi_l[i] = ldt(a.twi + i);
+  for (int i = threadIdx.x; i < ntf; i += nthr) stl(twf_l, i, ldt(a.twf + i));
+  for (int i = threadIdx.x; i < nti; i += nthr) stl(twi_l, i, ldt(a.twi + i));
 
   const int it_cur = st->it;
   const int nw = st->nw;
@@ -775,8 +800,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
 
   for (int r = threadIdx.x; r < a.n; r += nthr) {
-    ramp_l[r] = ldt(a.ramp[a.axis] + (size_t)w * a.n + r);
-    if (FUSE) ramp2_l[r] = ldt(a.ramp[a.axis] + (size_t)w2 * a.n + r);
+    stl(ramp_l, r, ldt(a.ramp[a.axis] + (size_t)w * a.n + r));
+    if (FUSE) stl(ramp2_l, r, ldt(a.ramp[a.axis] + (size_t)w2 * a.n + r));
     pe_l[r] = ldt(a.pe[a.axis] + (size_t)w * a.n + r);
   }
   // per-column coordinates (axes other than a.axis)
@@ -873,7 +898,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   const int S = pl.nst;
   constexpr bool MID = MODE == MODE_MID;
   const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
-  const float2* tw = (MODE == MODE_INV) ? twi_l : twf_l;
+  lcf2 tw = (MODE == MODE_INV) ? twi_l : twf_l;
   constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
 
   // Each region below is its own radix switch, so the register allocator
