@@ -266,6 +266,35 @@ def dropin(sim, src, steps: int, workload: str):
             "e_primed_rel_l2_vs_resident": rel, "steps": steps}
 
 
+def pstd_rate(workload: str, nsteps: int, reps: int):
+    """PSTD (PSTD.m:79-127) on the workload's medium (absorption Im n = 0.02
+    added so that the time stepping is damped), `nsteps` steps from rest
+    (energy threshold 0: the cap ends the run), the stepping timed over `reps`
+    runs after one warm-up.  Algorithmic bytes per step: 16 B/voxel per FFT pass, 44 for
+    the update pass (T, E, E_prev, c1..c3 in; E_next out)."""
+    import torch
+    from wavesim_amd import PSTD
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, src = WORKLOADS[workload]()
+    popt = dict(opt, boundary_type="ARL", energy_threshold=0.0)
+    p = PSTD(np.asarray(nmap) + 0.02j, popt)
+    p.exec(src, max_iterations=nsteps)
+    ts, its = [], 0
+    for _ in range(max(reps, 1)):
+        _, st = p.exec(src, max_iterations=nsteps)
+        ts.append(st.run_time)
+        its = st.iterations
+    t = float(np.median(ts))
+    nvox = int(np.prod(p.N[:3]))
+    ndim = sum(1 for n in p.N[:3] if n > 1)
+    bpv = 16 * (2 * ndim - 2) + 44
+    p.close()
+    return {"mode": "pstd", "workload": workload, "grid": p.N, "steps": its, "s": round(t, 6),
+            "steps_per_s": round(its / t, 1), "bytes_per_voxel_step": bpv,
+            "hbm_gbps_algorithmic": round(bpv * nvox * its / t / 1e9, 1),
+            "timed": "wsim_pstd_run: the steps from rest, fields resident on the GPU"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +314,9 @@ def main():
                     help="measure plain streaming kernels on 1 GiB buffers and exit")
     ap.add_argument("--dropin", action="store_true",
                     help="time wsim_run_algorithm end to end (host arrays in and out) and exit")
+    ap.add_argument("--pstd", type=int, default=0,
+                    help="time this many PSTD steps (PSTD.m) on the workload's medium with absorption "
+                         "Im n = 0.02 and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the 1 GiB copy whose counter readings calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -303,6 +335,9 @@ def main():
     import torch
     from wavesim_amd import WaveSim, WaveSimVector, plans
     from wavesim_amd.workloads import WORKLOADS
+    if args.pstd > 0:
+        print(json.dumps(pstd_rate(args.workload, args.pstd, args.steps)), flush=True)
+        return
 
     vector = args.workload.startswith("sphere_vector")
     t_setup = time.perf_counter()

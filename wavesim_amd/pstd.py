@@ -32,6 +32,7 @@ class PSTDState:
     converged: bool = False
     diff_energy: np.ndarray = field(default_factory=lambda: np.zeros(0))
     time: float = 0.0
+    run_time: float = 0.0   # the time stepping alone (device-resident), seconds
     E_full: Optional[np.ndarray] = None
 
 
@@ -135,12 +136,15 @@ class PSTD:
         mi = self.max_iterations() if max_iterations is None else max_iterations
         n = C.c_int64()
         cv = C.c_int()
+        t1 = time.perf_counter()
         capi.check(L.wsim_pstd_run(h, float(mi), self.energy_threshold, C.byref(n), C.byref(cv)))
+        run_s = time.perf_counter() - t1
         E = np.empty(self.N, np.complex64, order="F")
         de = np.zeros(max(int(n.value), 1))
         capi.check(L.wsim_pstd_result(h, capi.ptr(E), de.ctypes.data_as(C.POINTER(C.c_double)), int(n.value)))
         K = int(n.value)
-        st = PSTDState(it=K + 1, iterations=K, converged=bool(cv.value), diff_energy=de[:K], E_full=E)
+        st = PSTDState(it=K + 1, iterations=K, converged=cv.value == 1, diff_energy=de[:K], E_full=E,
+                       run_time=run_s)
         A_next = default_source((K + 1) / self.iterations_per_cycle)   # PSTD.m:125-126
         ph = np.complex64(np.exp(-1j * np.angle(A_next)))
         Eout = np.ascontiguousarray(self.crop_field(E) * ph)

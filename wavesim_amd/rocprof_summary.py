@@ -35,6 +35,8 @@ _PASS = {  # (CONTIG, MODE, PRO, EPI) -> pass name
     ("true", "1", "0", "3"): "pass_inv_mix_fwd",
     ("true", "2", "1", "1"): "pass_mid_pro_epi",
     ("true", "2", "1", "2"): "pass_mid_pro_epi",
+    ("true", "1", "0", "4"): "pstd_inv_update",
+    ("true", "2", "1", "4"): "pstd_step_1d",
 }
 
 
