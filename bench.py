@@ -295,6 +295,57 @@ def pstd_rate(workload: str, nsteps: int, reps: int):
             "timed": "wsim_pstd_run: the steps from rest, fields resident on the GPU"}
 
 
+def f64_rate(workload: str, reps: int, profile_reps: int):
+    """The workload solved in complex128 (single_precision = false,
+    Simulation.m:31, 221-252) to the same energy threshold: iterations, time
+    (median of `reps` solves after one warm-up, fields resident), the
+    algorithmic HBM rate (twice the complex64 bytes: 16-byte elements), the
+    per-kernel times, and the relative L2 distance between the complex128 and
+    complex64 fields E' (the precision spread of the reference's two modes)."""
+    import torch
+    from wavesim_amd import WaveSim, WaveSimVector
+    from wavesim_amd.workloads import WORKLOADS
+    vector = workload.startswith("sphere_vector")
+    nmap, opt, source = WORKLOADS[workload]()
+    out = {"mode": "f64", "workload": workload}
+    fields = {}
+    for prec in ("c128", "c64"):
+        o = dict(opt, single_precision=(prec == "c64"))
+        sim = (WaveSimVector if vector else WaveSim)(nmap, o)
+        sv = sim.solver(1)
+        src = sim._prepare(source)
+        sv.set_source(0, src.positions, src.values)
+        mi = sim.max_iterations()
+        sv.solve(mi, sim.energy_threshold)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(reps, 1)):
+            t0 = time.perf_counter()
+            sv.solve(mi, sim.energy_threshold)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        its = sv.result(0).iterations
+        nvox = int(np.prod(sim.N[:3]))
+        C = sim.N[3]
+        ndim = sum(1 for n in sim.N[:3] if n > 1)
+        prof = sv.profile(profile_reps)
+        fused = any(name == "pass_inv_mix_fwd" for name, _, _ in prof)
+        bpv = bytes_per_voxel_iteration(ndim, C, fused) * (2 if prec == "c128" else 1)
+        out[prec] = {"grid": sim.N, "iterations": its, "s": round(t, 6), "iterations_per_s": round(its / t, 2),
+                     "bytes_per_voxel_iteration": bpv,
+                     "hbm_gbps_algorithmic": round(bpv * nvox * its / t / 1e9, 1),
+                     "kernels_us": {n: round(ms * 1e3, 2) for n, ms, _ in prof},
+                     "kernels_gbps": {n: round(b / ms / 1e6, 1) for n, ms, b in prof if b}}
+        fields[prec] = sv.field(0)
+        sv.close()
+        del sim
+    a, b = fields["c128"], fields["c64"].astype(np.complex128)
+    out["e_primed_rel_l2_c64_vs_c128"] = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(a.ravel()))
+    out["timed"] = "wsim_solver_solve to the energy threshold, fields resident on the GPU"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +365,8 @@ def main():
                     help="measure plain streaming kernels on 1 GiB buffers and exit")
     ap.add_argument("--dropin", action="store_true",
                     help="time wsim_run_algorithm end to end (host arrays in and out) and exit")
+    ap.add_argument("--f64", action="store_true",
+                    help="solve the workload in complex128 and in complex64 and report both (JSON line)")
     ap.add_argument("--pstd", type=int, default=0,
                     help="time this many PSTD steps (PSTD.m) on the workload's medium with absorption "
                          "Im n = 0.02 and exit")
@@ -337,6 +390,9 @@ def main():
     from wavesim_amd.workloads import WORKLOADS
     if args.pstd > 0:
         print(json.dumps(pstd_rate(args.workload, args.pstd, args.steps)), flush=True)
+        return
+    if args.f64:
+        print(json.dumps(f64_rate(args.workload, args.steps, args.profile_reps)), flush=True)
         return
 
     vector = args.workload.startswith("sphere_vector")

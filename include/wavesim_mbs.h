@@ -37,6 +37,10 @@ typedef struct {
   float re, im;
 } wsim_cfloat;
 
+typedef struct {
+  double re, im;
+} wsim_cdouble;
+
 /* One wiggle descriptor (WaveSimBase.m:300-316): Fourier coordinates already
  * shifted by a quarter pixel and divided by sqrt(epsilon), and the real-space
  * phase ramps.  pxe/gx have Nx entries, pye/gy Ny, pze/gz Nz. */
@@ -223,6 +227,53 @@ int wsim_solver_clear_warm_start(wsim_solver* s);
 int wsim_solver_set_reference_energy(wsim_solver* s, int rhs, double energy);
 int wsim_solver_destroy(wsim_solver* s);
 
+/* ---- complex128 solvers ---------------------------------------------------
+ * The reference runs in complex128 when single_precision is false
+ * (Simulation.m:31, data_array Simulation.m:221-252): gamma, the wiggle
+ * tables, the sources and the fields are double, and the loop of
+ * WaveSimBase.m:157-197 runs in double on the GPU (its MATLAB branch; the
+ * MEX of wsim_run_algorithm takes single-precision arrays only,
+ * WaveSimBase.m:139-146).  These entry points create such a solver; solve,
+ * result, device_ptr, profile, tune, plans, reference energy, groups and
+ * destroy are the wsim_solver_* functions above (device pointers then address
+ * complex128 elements).  The complex64 set_source / get_field / propagate /
+ * set_medium / set_warm_start refuse a complex128 solver (WSIM_ERR_ARG or
+ * WSIM_ERR_UNSUPPORTED); new media and warm starts are complex64 only.
+ * Grids: at most 2^28 voxels (one complex128 component below 4 GiB). */
+typedef struct {
+  const double* pxe;
+  const double* pye;
+  const double* pze;
+  const wsim_cdouble* gx;
+  const wsim_cdouble* gy;
+  const wsim_cdouble* gz;
+} wsim_wiggle_f64;
+
+typedef struct {
+  double pos[4];
+  int64_t size[4];
+  const wsim_cdouble* values;
+} wsim_source_patch_f64;
+
+typedef struct {
+  int64_t N[4];         /* [Ny Nx Nz Npol] */
+  int nrhs;
+  int nwiggles;
+  double epsilon;
+  wsim_cdouble k02e;
+  int64_t roi[2][4];    /* 1-based inclusive [start; end] of [y x z pol] */
+  int energy_interval;
+} wsim_solver_desc_f64;
+
+int wsim_solver_create_f64(wsim_solver** out, const wsim_solver_desc_f64* desc, const wsim_cdouble* gamma,
+                           const wsim_wiggle_f64* wiggles);
+/* patches scaled by i/eps/Nw in double (WaveSimBase.m:171-172) */
+int wsim_solver_set_source_f64(wsim_solver* s, int rhs, int npatches, const wsim_source_patch_f64* patches);
+int wsim_solver_get_field_f64(wsim_solver* s, int rhs, wsim_cdouble* E_out);
+int wsim_solver_propagate_f64(wsim_solver* s, int wiggle, const wsim_cdouble* in, wsim_cdouble* out);
+/* bytes of one complex element of the solver's fields: 8 (complex64) or 16 (complex128) */
+int wsim_solver_element_size(wsim_solver* s);
+
 /* ---- groups of independent problems -------------------------------------
  * Replaces a loop of exec calls over different simulations (the reference's
  * way of running several problems: examples/pulse2d.m:31-40,
@@ -288,10 +339,6 @@ typedef struct {
   wsim_cfloat* gy;
   wsim_cfloat* gz;
 } wsim_wiggle_out;
-
-typedef struct {
-  double re, im;
-} wsim_cdouble;
 
 int wsim_problem_query(const wsim_problem_desc* desc, wsim_problem_info* info);
 /* n: the refractive-index map (shape[0]*shape[1]*shape[2] complex values in

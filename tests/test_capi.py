@@ -97,6 +97,26 @@ def test_grid_size_limit_is_checked_before_any_device_work():
         rc, msg = create(1024, 1024, 513, comps)
         assert rc == -3 and b"voxels" in msg, (rc, msg)
 
+    # complex128 solvers: 16-byte elements, at most 2^28 voxels
+    dummy_d = (C.c_double * 2)()
+
+    def create_f64(ny, nx, nz, C_=1):
+        d = capi.desc_f64_t()
+        d.N[:] = [ny, nx, nz, C_]
+        d.nrhs, d.nwiggles, d.epsilon, d.energy_interval = 1, 1, 3.0, 8
+        d.roi[0][:] = [1, 1, 1, 1]
+        d.roi[1][:] = [ny + 1, nx, nz, C_]
+        h = C.c_void_p()
+        rc = L.wsim_solver_create_f64(C.byref(h), C.byref(d), C.cast(dummy_d, C.c_void_p), wig)
+        assert rc != 0
+        return rc, L.wsim_last_error()
+
+    for comps in (1, 3):
+        rc, msg = create_f64(1024, 512, 512, comps)  # 2^28 voxels
+        assert rc == -1 and b"roi" in msg, (rc, msg)
+        rc, msg = create_f64(1024, 512, 513, comps)
+        assert rc == -3 and b"voxels" in msg, (rc, msg)
+
 
 @pytest.mark.parametrize("builder,vector", [(cases.case_2d_disordered, False),
                                             (cases.case_3d_disordered, False),

@@ -30,6 +30,10 @@ ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
 VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 3), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
             (1, 1, 0, 4), (1, 2, 1, 4), (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
 COMPONENTS = (1, 3)
+# kernel families: 0 complex64, 1 complex64 compact, 2 complex128 (the MBS
+# passes only -- must match WSIM_FOR_EACH_VARIANT_F64)
+FAMILY_F64 = 2
+PSTD_EPILOGUE = 4
 
 
 def hipcc() -> str:
@@ -81,7 +85,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
              (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
         for c in COMPONENTS:
-            for k in (0, 1):
+            for k in (0, 1) if e == PSTD_EPILOGUE else (0, 1, FAMILY_F64):
                 name = f"v_{ct}_{m}_{p}_{e}_{c}_{k}"
                 units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
                               [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",

@@ -30,11 +30,13 @@ class cfloat(C.Structure):
     _fields_ = [("re", C.c_float), ("im", C.c_float)]
 
 
+# (also the layout of wsim_wiggle_f64: six pointers)
 class wiggle_t(C.Structure):
     _fields_ = [("pxe", C.c_void_p), ("pye", C.c_void_p), ("pze", C.c_void_p),
                 ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p)]
 
 
+# (also the layout of wsim_source_patch_f64)
 class patch_t(C.Structure):
     _fields_ = [("pos", C.c_double * 4), ("size", C.c_int64 * 4), ("values", C.c_void_p)]
 
@@ -47,6 +49,12 @@ class desc_t(C.Structure):
 
 class cdouble(C.Structure):
     _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class desc_f64_t(C.Structure):
+    _fields_ = [("N", C.c_int64 * 4), ("nrhs", C.c_int), ("nwiggles", C.c_int),
+                ("epsilon", C.c_double), ("k02e", cdouble), ("roi", (C.c_int64 * 4) * 2),
+                ("energy_interval", C.c_int)]
 
 
 class problem_desc_t(C.Structure):
@@ -84,6 +92,12 @@ SYMBOLS = {
     "wsim_solver_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(desc_t), C.c_void_p,
                                      C.POINTER(wiggle_t)]),
     "wsim_solver_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(patch_t)]),
+    "wsim_solver_create_f64": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(desc_f64_t), C.c_void_p,
+                                         C.POINTER(wiggle_t)]),
+    "wsim_solver_set_source_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(patch_t)]),
+    "wsim_solver_get_field_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "wsim_solver_propagate_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "wsim_solver_element_size": (C.c_int, [C.c_void_p]),
     "wsim_solver_solve": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, C.c_char_p,
                                     C.c_double, CALLBACK, C.c_void_p]),
     "wsim_solver_result": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),
@@ -141,13 +155,17 @@ def check(rc: int) -> int:
     return rc
 
 
-def c64(a) -> np.ndarray:
-    """complex64, MATLAB column-major (Fortran) order."""
-    return np.asfortranarray(np.asarray(a, dtype=np.complex64))
+def c64(a, dtype=np.complex64) -> np.ndarray:
+    """complex64 (or ``dtype``), MATLAB column-major (Fortran) order."""
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
 
 
-def f32(a) -> np.ndarray:
-    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
+def f32(a, dtype=np.float32) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel())
+
+
+def real_dtype(cdtype) -> type:
+    return np.float64 if np.dtype(cdtype) == np.complex128 else np.float32
 
 
 def ptr(a: np.ndarray) -> int:
@@ -157,14 +175,15 @@ def ptr(a: np.ndarray) -> int:
 class Wiggles:
     """Keeps the host tables of the wiggle descriptors alive for a call."""
 
-    def __init__(self, wiggles):
+    def __init__(self, wiggles, dtype=np.complex64):
         self.keep = []
         arr = (wiggle_t * len(wiggles))()
+        rdt = real_dtype(dtype)
         for i, w in enumerate(wiggles):
-            t = [f32(w["pxe"]), f32(w["pye"]), f32(w["pze"]),
-                 np.ascontiguousarray(np.asarray(w["gx"], np.complex64).ravel()),
-                 np.ascontiguousarray(np.asarray(w["gy"], np.complex64).ravel()),
-                 np.ascontiguousarray(np.asarray(w["gz"], np.complex64).ravel())]
+            t = [f32(w["pxe"], rdt), f32(w["pye"], rdt), f32(w["pze"], rdt),
+                 np.ascontiguousarray(np.asarray(w["gx"], dtype).ravel()),
+                 np.ascontiguousarray(np.asarray(w["gy"], dtype).ravel()),
+                 np.ascontiguousarray(np.asarray(w["gz"], dtype).ravel())]
             self.keep += t
             arr[i] = wiggle_t(*[ptr(x) for x in t])
         self.arr = arr
@@ -174,14 +193,14 @@ class Wiggles:
 class Patches:
     """Source patches: list of (1-based position[4], value array in [y,x,z,pol] order)."""
 
-    def __init__(self, positions, values):
+    def __init__(self, positions, values, dtype=np.complex64):
         self.keep = []
         n = len(positions)
         arr = (patch_t * max(n, 1))()
         for i, (p, v) in enumerate(zip(positions, values)):
             v = np.asarray(v)
             v4 = v.reshape(list(v.shape) + [1] * (4 - v.ndim)) if v.ndim < 4 else v
-            vf = c64(v4)
+            vf = c64(v4, dtype)
             self.keep.append(vf)
             arr[i].pos = (C.c_double * 4)(*[float(x) for x in p])
             arr[i].size = (C.c_int64 * 4)(*[int(x) for x in v4.shape])

@@ -9,50 +9,74 @@
 
 namespace wsim {
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// complex values are float2 (complex64) or double2 (complex128); every
+// helper and butterfly below is written once for both
+template <class V> struct RealOf;
+template <> struct RealOf<float2> { using type = float; };
+template <> struct RealOf<double2> { using type = double; };
+template <class V> using real_of = typename RealOf<V>::type;
+template <class S> struct VecOf;
+template <> struct VecOf<float> { using type = float2; };
+template <> struct VecOf<double> { using type = double2; };
+template <class S> using vec_of = typename VecOf<S>::type;
+
+template <class V>
+__device__ __forceinline__ V mkv(real_of<V> x, real_of<V> y) {
+  V r;
+  r.x = x;
+  r.y = y;
+  return r;
 }
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+template <class V>
+__device__ __forceinline__ V cadd(V a, V b) { return mkv<V>(a.x + b.x, a.y + b.y); }
+template <class V>
+__device__ __forceinline__ V csub(V a, V b) { return mkv<V>(a.x - b.x, a.y - b.y); }
+template <class V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  return mkv<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class V>
+__device__ __forceinline__ V cconj(V a) { return mkv<V>(a.x, -a.y); }
+template <class V>
+__device__ __forceinline__ V cscale(V a, real_of<V> s) { return mkv<V>(a.x * s, a.y * s); }
 // a * (DIR * i):  DIR=-1 -> multiply by -i, DIR=+1 -> multiply by +i
-template <int DIR>
-__device__ __forceinline__ float2 mul_dir_i(float2 a) {
-  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+template <int DIR, class V>
+__device__ __forceinline__ V mul_dir_i(V a) {
+  return DIR < 0 ? mkv<V>(a.y, -a.x) : mkv<V>(-a.y, a.x);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft2(float2* v) {
-  float2 a = v[0], b = v[1];
+template <int DIR, class V>
+__device__ __forceinline__ void dft2(V* v) {
+  V a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
-  float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-  float2 s13 = cadd(a1, a3), d13 = mul_dir_i<DIR>(csub(a1, a3));
+template <int DIR, class V>
+__device__ __forceinline__ void dft4(V& a0, V& a1, V& a2, V& a3) {
+  V s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  V s13 = cadd(a1, a3), d13 = mul_dir_i<DIR>(csub(a1, a3));
   a0 = cadd(s02, s13);
   a2 = csub(s02, s13);
   a1 = cadd(d02, d13);
   a3 = csub(d02, d13);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft4(float2* v) { dft4<DIR>(v[0], v[1], v[2], v[3]); }
+template <int DIR, class V>
+__device__ __forceinline__ void dft4(V* v) { dft4<DIR>(v[0], v[1], v[2], v[3]); }
 
-template <int DIR>
-__device__ __forceinline__ void dft8(float2* v) {
-  constexpr float H = 0.70710678118654752f;
-  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+template <int DIR, class V>
+__device__ __forceinline__ void dft8(V* v) {
+  using S = real_of<V>;
+  const S H = S(0.70710678118654752440);
+  V e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  V o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
   dft4<DIR>(e0, e1, e2, e3);
   dft4<DIR>(o0, o1, o2, o3);
   // w8^k = exp(DIR * 2 pi i k / 8)
-  float2 t1 = make_float2(H * (o1.x - DIR * o1.y), H * (o1.y + DIR * o1.x));   // o1 * H(1 + DIR i)
-  float2 t2 = mul_dir_i<DIR>(o2);                                              // o2 * DIR i
-  float2 t3 = make_float2(H * (-o3.x - DIR * o3.y), H * (-o3.y + DIR * o3.x)); // o3 * H(-1 + DIR i)
+  V t1 = mkv<V>(H * (o1.x - DIR * o1.y), H * (o1.y + DIR * o1.x));   // o1 * H(1 + DIR i)
+  V t2 = mul_dir_i<DIR>(o2);                                          // o2 * DIR i
+  V t3 = mkv<V>(H * (-o3.x - DIR * o3.y), H * (-o3.y + DIR * o3.x)); // o3 * H(-1 + DIR i)
   v[0] = cadd(e0, o0);
   v[4] = csub(e0, o0);
   v[1] = cadd(e1, t1);
@@ -63,10 +87,11 @@ __device__ __forceinline__ void dft8(float2* v) {
   v[7] = csub(e3, t3);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft16(float2* v) {
+template <int DIR, class V>
+__device__ __forceinline__ void dft16(V* v) {
+  using S = real_of<V>;
   // 4 x 4 decomposition: n = 4 n1 + n2, k = k1 + 4 k2
-  float2 a[4][4];
+  V a[4][4];
 #pragma unroll
   for (int n2 = 0; n2 < 4; ++n2) {
     a[n2][0] = v[n2];
@@ -76,27 +101,27 @@ __device__ __forceinline__ void dft16(float2* v) {
     dft4<DIR>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);  // over n1 -> k1
   }
   // twiddle w16^(n2 k1)
-  constexpr float C1 = 0.92387953251128676f, S1 = 0.38268343236508977f, H = 0.70710678118654752f;
-  const float cs[4] = {1.f, C1, H, S1};
-  const float sn[4] = {0.f, S1, H, C1};
+  const S C1 = S(0.92387953251128676), S1 = S(0.38268343236508977), H = S(0.70710678118654752440);
+  const S cs[4] = {S(1), C1, H, S1};
+  const S sn[4] = {S(0), S1, H, C1};
 #pragma unroll
   for (int n2 = 1; n2 < 4; ++n2) {
 #pragma unroll
     for (int k1 = 1; k1 < 4; ++k1) {
       const int m = n2 * k1;  // 1..9
       // exp(DIR i 2 pi m / 16)
-      float c, s;
+      S c, s;
       if (m < 4) { c = cs[m]; s = sn[m]; }
-      else if (m == 4) { c = 0.f; s = 1.f; }
+      else if (m == 4) { c = S(0); s = S(1); }
       else if (m < 8) { c = -cs[8 - m]; s = sn[8 - m]; }
-      else if (m == 8) { c = -1.f; s = 0.f; }
+      else if (m == 8) { c = S(-1); s = S(0); }
       else { c = -cs[m - 8]; s = -sn[m - 8]; }
-      a[n2][k1] = cmul(a[n2][k1], make_float2(c, DIR * s));
+      a[n2][k1] = cmul(a[n2][k1], mkv<V>(c, DIR * s));
     }
   }
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
-    float2 b0 = a[0][k1], b1 = a[1][k1], b2 = a[2][k1], b3 = a[3][k1];
+    V b0 = a[0][k1], b1 = a[1][k1], b2 = a[2][k1], b3 = a[3][k1];
     dft4<DIR>(b0, b1, b2, b3);  // over n2 -> k2
     v[k1] = b0;
     v[k1 + 4] = b1;
@@ -108,70 +133,71 @@ __device__ __forceinline__ void dft16(float2* v) {
 // cos/sin(2 pi m / R) for m = 1..(R-1)/2, odd R
 template <int R> struct OddTrig;
 template <> struct OddTrig<3> {
-  static constexpr float c[1] = {-0.49999999999999978f};
-  static constexpr float s[1] = {0.86602540378443871f};
+  static constexpr double c[1] = {-0.49999999999999978};
+  static constexpr double s[1] = {0.86602540378443871};
 };
 template <> struct OddTrig<5> {
-  static constexpr float c[2] = {0.30901699437494745f, -0.80901699437494734f};
-  static constexpr float s[2] = {0.95105651629515353f, 0.58778525229247325f};
+  static constexpr double c[2] = {0.30901699437494745, -0.80901699437494734};
+  static constexpr double s[2] = {0.95105651629515353, 0.58778525229247325};
 };
 template <> struct OddTrig<7> {
-  static constexpr float c[3] = {0.62348980185873359f, -0.22252093395631434f, -0.90096886790241903f};
-  static constexpr float s[3] = {0.7818314824680298f, 0.97492791218182362f, 0.43388373911755823f};
+  static constexpr double c[3] = {0.62348980185873359, -0.22252093395631434, -0.90096886790241903};
+  static constexpr double s[3] = {0.7818314824680298, 0.97492791218182362, 0.43388373911755823};
 };
 template <> struct OddTrig<11> {
-  static constexpr float c[5] = {0.84125353283118121f, 0.41541501300188644f, -0.142314838273285f,
-                                 -0.65486073394528499f, -0.95949297361449737f};
-  static constexpr float s[5] = {0.54064081745559756f, 0.90963199535451833f, 0.9898214418809328f,
-                                 0.75574957435425827f, 0.28173255684142967f};
+  static constexpr double c[5] = {0.84125353283118121, 0.41541501300188644, -0.142314838273285,
+                                 -0.65486073394528499, -0.95949297361449737};
+  static constexpr double s[5] = {0.54064081745559756, 0.90963199535451833, 0.9898214418809328,
+                                 0.75574957435425827, 0.28173255684142967};
 };
 
 // cos(2 pi m / R) and sin(2 pi m / R) for any m (reduced mod R)
 template <int R>
-__device__ __forceinline__ constexpr float odd_cos(int m) {
+__device__ __forceinline__ constexpr double odd_cos(int m) {
   m %= R;
-  return m == 0 ? 1.f : (m <= (R - 1) / 2 ? OddTrig<R>::c[m - 1] : OddTrig<R>::c[R - m - 1]);
+  return m == 0 ? 1.0 : (m <= (R - 1) / 2 ? OddTrig<R>::c[m - 1] : OddTrig<R>::c[R - m - 1]);
 }
 template <int R>
-__device__ __forceinline__ constexpr float odd_sin(int m) {
+__device__ __forceinline__ constexpr double odd_sin(int m) {
   m %= R;
-  return m == 0 ? 0.f : (m <= (R - 1) / 2 ? OddTrig<R>::s[m - 1] : -OddTrig<R>::s[R - m - 1]);
+  return m == 0 ? 0.0 : (m <= (R - 1) / 2 ? OddTrig<R>::s[m - 1] : -OddTrig<R>::s[R - m - 1]);
 }
 
-template <int R, int DIR>
-__device__ __forceinline__ void dft_odd(float2* v) {
+template <int R, int DIR, class V>
+__device__ __forceinline__ void dft_odd(V* v) {
+  using S = real_of<V>;
   constexpr int H = (R - 1) / 2;
-  float2 a[H], b[H];
+  V a[H], b[H];
 #pragma unroll
   for (int n = 1; n <= H; ++n) {
     a[n - 1] = cadd(v[n], v[R - n]);
     b[n - 1] = csub(v[n], v[R - n]);
   }
-  float2 x0 = v[0];
-  float2 sum = x0;
+  V x0 = v[0];
+  V sum = x0;
 #pragma unroll
   for (int n = 0; n < H; ++n) sum = cadd(sum, a[n]);
   v[0] = sum;
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
-    float2 t1 = x0, t2 = make_float2(0.f, 0.f);
+    V t1 = x0, t2 = mkv<V>(S(0), S(0));
 #pragma unroll
     for (int n = 1; n <= H; ++n) {
-      const float c = odd_cos<R>(n * k), s = odd_sin<R>(n * k);
+      const S c = S(odd_cos<R>(n * k)), s = S(odd_sin<R>(n * k));
       t1.x += a[n - 1].x * c;
       t1.y += a[n - 1].y * c;
       t2.x += b[n - 1].x * s;
       t2.y += b[n - 1].y * s;
     }
     // forward: X[k] = t1 - i t2, X[R-k] = t1 + i t2 ; inverse swaps the sign
-    float2 it2 = mul_dir_i<DIR>(t2);  // DIR*i*t2
+    V it2 = mul_dir_i<DIR>(t2);  // DIR*i*t2
     v[k] = cadd(t1, it2);
     v[R - k] = csub(t1, it2);
   }
 }
 
-template <int R, int DIR>
-__device__ __forceinline__ void dft(float2* v) {
+template <int R, int DIR, class V>
+__device__ __forceinline__ void dft(V* v) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);

@@ -53,6 +53,15 @@ static void require(bool ok, const std::string& msg, int code = WSIM_ERR_ARG) {
   if (!ok) throw Error(code, msg);
 }
 
+// a float2 / double2 on the host
+template <class V>
+static V mkv_host(real_of<V> x, real_of<V> y) {
+  V r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+
 // ----------------------------------------------------------------------------
 // index arithmetic
 // ----------------------------------------------------------------------------
@@ -125,7 +134,8 @@ static std::vector<int> default_radices(int n, int C = 1) {
 
 struct AxisTables {
   AxisPlan plan;
-  std::vector<float2> twf, twi;
+  std::vector<float2> twf, twi;      // complex64 solvers
+  std::vector<double2> twf_d, twi_d; // complex128 solvers (same angles)
 };
 
 static AxisTables build_axis(int n, const std::vector<int>& radices) {
@@ -134,7 +144,8 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
   p.n = n;
   p.nst = (int)radices.size();
   require(p.nst <= MAXST, "too many FFT stages", WSIM_ERR_UNSUPPORTED);
-  auto fill = [&](Stage* st, std::vector<float2>& tw, const std::vector<int>& rad, int sign) {
+  auto fill = [&](Stage* st, std::vector<float2>& tw, std::vector<double2>& twd, const std::vector<int>& rad,
+                  int sign) {
     int Ns = 1;
     for (size_t s = 0; s < rad.size(); ++s) {
       const int R = rad[s];
@@ -150,15 +161,18 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
         for (int k = 0; k < Ns; ++k) {
           const double ang = sign * 2.0 * M_PI * (double)r * (double)k / ((double)Ns * R);
           tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+          twd.push_back(make_double2(std::cos(ang), std::sin(ang)));
         }
       Ns *= R;
     }
   };
   std::vector<int> rev(radices.rbegin(), radices.rend());
-  fill(p.f, t.twf, radices, -1);
-  fill(p.i, t.twi, rev, +1);
+  fill(p.f, t.twf, t.twf_d, radices, -1);
+  fill(p.i, t.twi, t.twi_d, rev, +1);
   if (t.twf.empty()) t.twf.push_back(make_float2(1.f, 0.f));
   if (t.twi.empty()) t.twi.push_back(make_float2(1.f, 0.f));
+  if (t.twf_d.empty()) t.twf_d.push_back(make_double2(1.0, 0.0));
+  if (t.twi_d.empty()) t.twi_d.push_back(make_double2(1.0, 0.0));
   return t;
 }
 
@@ -167,7 +181,9 @@ static AxisTables build_axis(int n, const std::vector<int>& radices) {
 constexpr int kControlThreads = 1024;
 constexpr int kXcdAutoBlocks = 2048;  // automatic XCD tile ranges up to this many workgroups per rhs
 // largest grid: byte offsets 8 * (nvox - 1) of one component fit 32 bits
+// (16 * (nvox - 1) for complex128 fields)
 constexpr long long kMaxVoxels = (1ll << 29);
+constexpr long long kMaxVoxelsF64 = (1ll << 28);
 
 // FFT plans tuned on MI355X for given problem shapes, applied when a solver of
 // that shape is created (the rows are generated from tuned_plans.json by the
@@ -331,7 +347,8 @@ __global__ void __launch_bounds__(kControlThreads) control_pstd_kernel(DevState*
 }
 
 // x[c][v] *= gamma[v] for the C components of one field (warm-start seeding)
-__global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2* g, long long nvox, int C) {
+template <class V>
+__global__ void __launch_bounds__(256) gamma_mul_kernel(V* x, const V* g, long long nvox, int C) {
   const long long n = nvox * C;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     x[i] = cmul(x[i], g[i % nvox]);
@@ -339,8 +356,8 @@ __global__ void __launch_bounds__(256) gamma_mul_kernel(float2* x, const float2*
 
 // sparse source upload: the entries of a source that touch few voxels,
 // unique indices, written into a field that is zero elsewhere
-__global__ void __launch_bounds__(256) scatter_kernel(float2* S, const unsigned long long* idx, const float2* val,
-                                                     long long m) {
+template <class V>
+__global__ void __launch_bounds__(256) scatter_kernel(V* S, const unsigned long long* idx, const V* val, long long m) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256)
     S[idx[i]] = val[i];
 }
@@ -348,13 +365,13 @@ __global__ void __launch_bounds__(256) scatter_kernel(float2* S, const unsigned 
 // The mix epilogue adds the increments to E' two at a time, on even
 // iterations; a rhs whose last iteration was odd still lacks its last dE'.
 // One block row per rhs; blocks of a rhs that needs nothing return at once.
-__global__ void __launch_bounds__(256) finish_e_kernel(const DevState* st, float2* E, const float2* dE,
-                                                      long long fe) {
+template <class V>
+__global__ void __launch_bounds__(256) finish_e_kernel(const DevState* st, V* E, const V* dE, long long fe) {
   const int r = blockIdx.y;
   const int last = st->stopped[r] ? st->n_iter[r] : st->it - 1;
   if ((last & 1) == 0) return;
-  float2* e = E + (size_t)r * fe;
-  const float2* d = dE + (size_t)r * fe;
+  V* e = E + (size_t)r * fe;
+  const V* d = dE + (size_t)r * fe;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < fe; i += (long long)gridDim.x * 256)
     e[i] = cadd(e[i], d[i]);
 }
@@ -364,7 +381,17 @@ __global__ void __launch_bounds__(256) finish_e_kernel(const DevState* st, float
 // ----------------------------------------------------------------------------
 constexpr size_t LDS_CAP = 80 * 1024;  // two workgroups per 160 KB CU
 
-static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C, bool compact) {
+// family: 0 complex64, 1 complex64 compact, kFamilyF64 complex128
+static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C, int family) {
+  if (family == kFamilyF64) {
+#define WSIM_SELECT_F64(ct, m, p, e)                                   \
+  if ((contig ? 1 : 0) == ct && mode == m && pro == p && epi == e)     \
+    return C == 1 ? wsim_kptr_##ct##_##m##_##p##_##e##_1_2() : wsim_kptr_##ct##_##m##_##p##_##e##_3_2();
+    WSIM_FOR_EACH_VARIANT_F64(WSIM_SELECT_F64)
+#undef WSIM_SELECT_F64
+    throw Error(WSIM_ERR_UNSUPPORTED, "no complex128 kernel for this pass variant");
+  }
+  const bool compact = family == 1;
 #define WSIM_SELECT(ct, m, p, e)                                                                       \
   if ((contig ? 1 : 0) == ct && mode == m && pro == p && epi == e)                                     \
     return C == 1 ? (compact ? wsim_kptr_##ct##_##m##_##p##_##e##_1_1() : wsim_kptr_##ct##_##m##_##p##_##e##_1_0()) \
@@ -376,24 +403,25 @@ static KernelFn select_kernel(bool contig, int mode, int pro, int epi, int C, bo
 
 // LDS in front of the transform data: column table and the per-row ramp and
 // coordinate tables (layout of axis_pass_kernel)
-static size_t lds_header(int ncol, int n, int ntw, bool fuse = false) {
+static size_t lds_header(int ncol, int n, int ntw, bool fuse, bool f64) {
   auto r16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  return r16(ncol * sizeof(ColInfo)) + (fuse ? 2 : 1) * r16((size_t)n * sizeof(float2)) +
-         r16((size_t)n * sizeof(float)) + (size_t)ntw * sizeof(float2);
+  const size_t ci = f64 ? sizeof(ColInfoT<double>) : sizeof(ColInfo);
+  const size_t cz = f64 ? sizeof(double2) : sizeof(float2), rz = f64 ? sizeof(double) : sizeof(float);
+  return r16(ncol * ci) + (fuse ? 2 : 1) * r16((size_t)n * cz) + r16((size_t)n * rz) + (size_t)ntw * cz;
 }
 
 // contig LDS line pitch: one padding slot every 8 rows, plus one
 static int contig_pitch(int n) { return n + (n >> WSIM_PAD_SHIFT) + 1; }
 
-static int its_for(int C, int R, bool compact = false) { return its_per_stage(ept_for(C, compact ? 1 : 0), C, R); }
+static int its_for(int C, int R, int family = 0) { return its_per_stage(ept_for(C, family), C, R); }
 
 // every stage of the plan covers ncol lines of length n within the
 // butterflies-per-thread budget of the kernel family
-static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, bool compact, int NT) {
+static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, int family, int NT) {
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.f[s].R;
     const long long need = ((long long)ncol * (n / R) + NT - 1) / NT;
-    if (need > its_for(C, R, compact)) return false;
+    if (need > its_for(C, R, family)) return false;
   }
   return true;
 }
@@ -418,6 +446,13 @@ struct Solver {
   long long nvox = 0;
   float epsilon = 1.f;
   float2 k02e{0.f, 0.f};
+  // complex128 solver (Simulation.m:221-252 with single_precision = false):
+  // every field, table and twiddle buffer below holds double2 / double
+  // elements (the pointers keep their complex64 types; cz() is the element
+  // size), and epsilon and k02e are kept in double
+  bool f64 = false;
+  double epsilon_d = 1.0;
+  double2 k02e_d{0.0, 0.0};
   int roi_lo[4], roi_hi[4];
   std::vector<int> axes;  // non-trivial axes in [y x z] order
   int dev = 0;
@@ -505,21 +540,32 @@ struct Solver {
   }
 
   long long field_elems() const { return (long long)C * nvox; }
+  // bytes of one complex / real element of the fields and tables
+  size_t cz() const { return f64 ? sizeof(double2) : sizeof(float2); }
+  size_t rz() const { return f64 ? sizeof(double) : sizeof(float); }
+  // element k of a complex buffer
+  float2* at(float2* p, size_t k) const { return reinterpret_cast<float2*>(reinterpret_cast<char*>(p) + k * cz()); }
+  int family_base() const { return f64 ? kFamilyF64 : 0; }
 
   template <class T_>
   void dalloc(T_** p, size_t n) {
-    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T_);
+    dalloc_bytes(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T_));
+  }
+  // complex and real buffers of n elements of the solver's precision
+  void dalloc_c(float2** p, size_t n) { dalloc_bytes(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * cz()); }
+  void dalloc_r(float** p, size_t n) { dalloc_bytes(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * rz()); }
+  void dalloc_bytes(void** p, size_t bytes) {
     if (arena) {
       const size_t b = (bytes + 255) & ~(size_t)255;
       require(arena_used + b <= arena_cap,
               "problem needs more than the " + std::to_string(arena_cap >> 20) +
                   " MiB scratch of the allocation-free entry point",
               WSIM_ERR_UNSUPPORTED);
-      *p = reinterpret_cast<T_*>(arena + arena_used);
+      *p = arena + arena_used;
       arena_used += b;
       return;
     }
-    HIPCHK(hipMalloc((void**)p, bytes));
+    HIPCHK(hipMalloc(p, bytes));
   }
   // frees what dalloc allocated (arena memory is the caller's: nothing to do)
   void dfree(void* p) {
@@ -546,13 +592,13 @@ struct Solver {
     a.plan = pl;
     size_t lds_cap = lds_limit;
     auto fits = [&](int ncol) {
-      if (!tile_fits(pl, n, ncol, C, false, nthr)) return false;
+      if (!tile_fits(pl, n, ncol, C, family_base(), nthr)) return false;
       size_t bytes;
       if (contig)
-        bytes = (size_t)C * ncol * contig_pitch(n) * sizeof(float2);
+        bytes = (size_t)C * ncol * contig_pitch(n) * cz();
       else
-        bytes = (size_t)n * (C * ncol + 1) * sizeof(float2);
-      bytes += lds_header(ncol, n, ntw, fuse);
+        bytes = (size_t)n * (C * ncol + 1) * cz();
+      bytes += lds_header(ncol, n, ntw, fuse, f64);
       return bytes <= lds_cap;
     };
     // long axes whose single-column tile exceeds the target take up to the
@@ -603,10 +649,10 @@ struct Solver {
       blocks = (int)(a.ntile_inner * outer);
     }
     if (contig)
-      lds = (size_t)C * a.ncol * contig_pitch(n) * sizeof(float2);
+      lds = (size_t)C * a.ncol * contig_pitch(n) * cz();
     else
-      lds = (size_t)n * (C * a.ncol + 1) * sizeof(float2);
-    lds += lds_header(a.ncol, n, ntw, fuse);
+      lds = (size_t)n * (C * a.ncol + 1) * cz();
+    lds += lds_header(a.ncol, n, ntw, fuse, f64);
   }
 
   PassArgs base_args() const {
@@ -622,6 +668,12 @@ struct Solver {
     const double kr = k02e.x, ki = k02e.y, kk = kr * kr + ki * ki;
     a.inv_k02e = make_float2((float)(kr / kk), (float)(-ki / kk));
     a.invN = (float)(1.0 / (double)nvox);
+    {
+      const double dr = k02e_d.x, di = k02e_d.y, dd = dr * dr + di * di;
+      a.k02e_d = k02e_d;
+      a.inv_k02e_d = make_double2(dr / dd, -di / dd);
+      a.invN_d = 1.0 / (double)nvox;
+    }
     for (int d = 0; d < 4; ++d) {
       a.roi_lo[d] = roi_lo[d];
       a.roi_hi[d] = roi_hi[d];
@@ -639,8 +691,9 @@ struct Solver {
     configure(L.args, contig, mode, axis, blocks, L.lds, epi == EPI_MIX_FWD);
     // the compact register family whenever the tile fits it (WSIM_COMPACT=0 disables)
     const char* env_k = std::getenv("WSIM_COMPACT");
-    const bool compact = !(env_k && std::atoi(env_k) == 0) && tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, true, threads_of(axis, contig));
-    L.fn = select_kernel(contig, mode, pro, epi, C, compact);
+    const bool compact = !f64 && !(env_k && std::atoi(env_k) == 0) &&
+                         tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, 1, threads_of(axis, contig));
+    L.fn = select_kernel(contig, mode, pro, epi, C, f64 ? kFamilyF64 : (compact ? 1 : 0));
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
@@ -657,7 +710,7 @@ struct Solver {
                         : epi == EPI_MIX_FWD ? 40.0 * C + 8.0
                         : epi == EPI_PSTD ? 32.0 * C + 12.0
                                           : 16.0 * C;
-    L.bytes = perv * (double)nvox * nrhs;
+    L.bytes = perv * (double)nvox * nrhs * (f64 ? 2.0 : 1.0);
     if (L.lds > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)L.lds));
@@ -799,22 +852,30 @@ struct Solver {
     HIPCHK(hipGetLastError());
   }
 
-  // per-wiggle coordinate and ramp tables (WaveSimBase.m:300-316)
-  void upload_wiggles(const wsim_wiggle* wig) {
+  // per-wiggle coordinate and ramp tables (WaveSimBase.m:300-316):
+  // wsim_wiggle (complex64 solvers) or wsim_wiggle_f64 (complex128)
+  void upload_wiggles(const void* wig) {
+    if (f64)
+      upload_wiggles_t<double, double2>(static_cast<const wsim_wiggle_f64*>(wig));
+    else
+      upload_wiggles_t<float, float2>(static_cast<const wsim_wiggle*>(wig));
+  }
+  template <class R, class V, class W>
+  void upload_wiggles_t(const W* wig) {
     for (int a = 0; a < 3; ++a) {
-      std::vector<float> pe((size_t)nw * N[a]);
-      std::vector<float2> rp((size_t)nw * N[a]);
+      std::vector<R> pe((size_t)nw * N[a]);
+      std::vector<V> rp((size_t)nw * N[a]);
       for (int w = 0; w < nw; ++w) {
-        const float* src = a == 0 ? wig[w].pye : (a == 1 ? wig[w].pxe : wig[w].pze);
-        const wsim_cfloat* gr = a == 0 ? wig[w].gy : (a == 1 ? wig[w].gx : wig[w].gz);
+        const auto* src = a == 0 ? wig[w].pye : (a == 1 ? wig[w].pxe : wig[w].pze);
+        const auto* gr = a == 0 ? wig[w].gy : (a == 1 ? wig[w].gx : wig[w].gz);
         require(src != nullptr && gr != nullptr, "null wiggle table");
         for (int i = 0; i < N[a]; ++i) {
           pe[(size_t)w * N[a] + i] = src[i];
-          rp[(size_t)w * N[a] + i] = make_float2(gr[i].re, gr[i].im);
+          rp[(size_t)w * N[a] + i] = mkv_host<V>(gr[i].re, gr[i].im);
         }
       }
-      HIPCHK(hipMemcpy(d_pe[a], pe.data(), pe.size() * sizeof(float), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d_ramp[a], rp.data(), rp.size() * sizeof(float2), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_pe[a], pe.data(), pe.size() * sizeof(R), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_ramp[a], rp.data(), rp.size() * sizeof(V), hipMemcpyHostToDevice));
     }
   }
 
@@ -822,6 +883,7 @@ struct Solver {
   // (sources are scaled by i/eps/Nw when set: set them again afterwards)
   void set_medium(const wsim_cfloat* g, float eps, wsim_cfloat k, const wsim_wiggle* wig) {
     require(g != nullptr && wig != nullptr, "null argument");
+    require(!f64, "a complex128 solver takes no new medium: create another one", WSIM_ERR_UNSUPPORTED);
     upload_wiggles(wig);
     HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
     epsilon = eps;
@@ -841,10 +903,12 @@ struct Solver {
     dfree(d_twf[a]);
     dfree(d_twi[a]);
     d_twf[a] = d_twi[a] = nullptr;
-    dalloc(&d_twf[a], tab[a].twf.size());
-    dalloc(&d_twi[a], tab[a].twi.size());
-    HIPCHK(hipMemcpy(d_twf[a], tab[a].twf.data(), tab[a].twf.size() * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_twi[a], tab[a].twi.data(), tab[a].twi.size() * sizeof(float2), hipMemcpyHostToDevice));
+    dalloc_c(&d_twf[a], tab[a].twf.size());
+    dalloc_c(&d_twi[a], tab[a].twi.size());
+    const void* tf = f64 ? (const void*)tab[a].twf_d.data() : (const void*)tab[a].twf.data();
+    const void* ti = f64 ? (const void*)tab[a].twi_d.data() : (const void*)tab[a].twi.data();
+    HIPCHK(hipMemcpy(d_twf[a], tf, tab[a].twf.size() * cz(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_twi[a], ti, tab[a].twi.size() * cz(), hipMemcpyHostToDevice));
   }
 
   // (re)build the launches of one iteration; partials are sized for the
@@ -1088,7 +1152,9 @@ struct Solver {
     }
   }
 
-  void create(const wsim_solver_desc* d, const wsim_cfloat* g, const wsim_wiggle* wig) {
+  // g: nvox complex values of the solver's precision (wsim_cfloat or
+  // wsim_cdouble), wig: wsim_wiggle or wsim_wiggle_f64 tables
+  void create(const wsim_solver_desc* d, const void* g, const void* wig) {
     require(d != nullptr && (g != nullptr || pstd) && wig != nullptr, "null argument");
     for (int a = 0; a < 3; ++a) {
       require(d->N[a] >= 1 && d->N[a] < (1 << 30), "grid size out of range");
@@ -1108,8 +1174,9 @@ struct Solver {
     // from a per-workgroup scalar base (global_load/store with a 32-bit lane
     // offset): an element offset reaches nvox - 1 in the strided passes, so
     // one component's field must stay below 4 GiB
-    require(nvox <= kMaxVoxels, "grid of " + std::to_string(nvox) + " voxels: at most " +
-                                    std::to_string(kMaxVoxels) + " (one component below 4 GiB)",
+    const long long maxv = f64 ? kMaxVoxelsF64 : kMaxVoxels;
+    require(nvox <= maxv, "grid of " + std::to_string(nvox) + " voxels: at most " + std::to_string(maxv) +
+                              " (one component below 4 GiB)",
             WSIM_ERR_UNSUPPORTED);
     for (int c = 0; c < 4; ++c) {
       const long long lim = c < 3 ? N[c] : C;
@@ -1139,25 +1206,25 @@ struct Solver {
               "axis length " + std::to_string(N[a]) + " has a prime factor above 11",
               WSIM_ERR_UNSUPPORTED);
       build_tables(a);
-      dalloc(&d_pe[a], (size_t)nw * N[a]);
-      dalloc(&d_ramp[a], (size_t)nw * N[a]);
+      dalloc_r(&d_pe[a], (size_t)nw * N[a]);
+      dalloc_c(&d_ramp[a], (size_t)nw * N[a]);
     }
     upload_wiggles(wig);
     // fields
     const size_t fe = (size_t)field_elems() * nrhs;
     if (ext_gamma) {
-      gamma = reinterpret_cast<float2*>(const_cast<wsim_cfloat*>(g));  // the caller's device array
+      gamma = reinterpret_cast<float2*>(const_cast<void*>(g));  // the caller's device array
     } else if (pstd) {
       gamma = nullptr;  // PSTD has no gamma
     } else {
-      dalloc(&gamma, nvox);
-      HIPCHK(hipMemcpy(gamma, g, nvox * sizeof(float2), hipMemcpyHostToDevice));
+      dalloc_c(&gamma, nvox);
+      HIPCHK(hipMemcpy(gamma, g, nvox * cz(), hipMemcpyHostToDevice));
     }
-    if (!ext_E) dalloc(&E, fe);
-    dalloc(&dE, fe);
-    dalloc(&T, fe);
-    dalloc(&S, fe);
-    HIPCHK(hipMemset(S, 0, fe * sizeof(float2)));
+    if (!ext_E) dalloc_c(&E, fe);
+    dalloc_c(&dE, fe);
+    dalloc_c(&T, fe);
+    dalloc_c(&S, fe);
+    HIPCHK(hipMemset(S, 0, fe * cz()));
     has_source.assign(nrhs, 0);
     if (pstd) {
       dalloc(&coef, 3 * (size_t)nvox);
@@ -1204,22 +1271,35 @@ struct Solver {
   }
 
   void set_source(int rhs, int np, const wsim_source_patch* patches) {
+    require(!f64, "this solver computes in complex128: set its sources with wsim_solver_set_source_f64");
+    set_source_t<float2>(rhs, np, patches);
+  }
+  void set_source_f64(int rhs, int np, const wsim_source_patch_f64* patches) {
+    require(f64, "this solver computes in complex64: set its sources with wsim_solver_set_source");
+    set_source_t<double2>(rhs, np, patches);
+  }
+
+  // V = float2 (complex64 patches) or double2 (complex128 patches)
+  template <class V, class Patch>
+  void set_source_t(int rhs, int np, const Patch* patches) {
+    using R = real_of<V>;
     require(rhs >= 0 && rhs < nrhs, "rhs out of range");
     require(np >= 0 && (np == 0 || patches != nullptr), "null patches");
-    // source pre-multiplied by A = i/eps/Nw in single precision, the patches
+    // source pre-multiplied by A = i/eps/Nw in the field's precision, the patches
     // added in order into a zero field (WaveSimBase.m:171-172; Source.m:163-169)
     const size_t fe = (size_t)field_elems();
     // (PSTD: unscaled, the amplitude of each step multiplies it in the update)
-    const float ar = pstd ? 1.f : 0.f, ai = pstd ? 0.f : (1.0f / epsilon) / (float)nw;  // (1i / eps) / Nw
+    const R eps = f64 ? (R)epsilon_d : (R)epsilon;
+    const R ar = pstd ? R(1) : R(0), ai = pstd ? R(0) : (R(1) / eps) / (R)nw;  // (1i / eps) / Nw
     bool any = false;
     struct Entry {
       size_t g;
-      float2 v;
+      V v;
     };
     std::vector<Entry> ent;
     size_t total = 0;
     for (int p = 0; p < np; ++p) {
-      const wsim_source_patch& sp = patches[p];
+      const Patch& sp = patches[p];
       long long pos[4], sz[4];
       for (int c = 0; c < 4; ++c) {
         pos[c] = (long long)std::llround(sp.pos[c]) - 1;
@@ -1233,10 +1313,10 @@ struct Solver {
     // few voxels: upload only those (a point source at 320^3 would otherwise
     // move a 262 MB dense field through host memory and PCIe)
     const bool sparse = total * 4 < fe;
-    std::vector<float2> h(sparse ? 0 : fe, make_float2(0.f, 0.f));
+    std::vector<V> h(sparse ? 0 : fe, mkv_host<V>(R(0), R(0)));
     if (sparse) ent.reserve(total);
     for (int p = 0; p < np; ++p) {
-      const wsim_source_patch& sp = patches[p];
+      const Patch& sp = patches[p];
       long long pos[4], sz[4];
       for (int c = 0; c < 4; ++c) {
         pos[c] = (long long)std::llround(sp.pos[c]) - 1;
@@ -1247,31 +1327,31 @@ struct Solver {
         for (long long z = 0; z < sz[2]; ++z)
           for (long long x = 0; x < sz[1]; ++x)
             for (long long y = 0; y < sz[0]; ++y, ++k) {
-              const wsim_cfloat v = sp.values[k];
+              const auto v = sp.values[k];
               const size_t g = (size_t)(pos[3] + q) * nvox +
                                (size_t)((pos[2] + z) * N[1] + (pos[1] + x)) * N[0] + (size_t)(pos[0] + y);
               // A * val with A = (ar + i ai)
-              const float re = ar * v.re - ai * v.im;
-              const float im = ar * v.im + ai * v.re;
+              const R re = ar * v.re - ai * v.im;
+              const R im = ar * v.im + ai * v.re;
               if (sparse) {
-                ent.push_back({g, make_float2(re, im)});
+                ent.push_back({g, mkv_host<V>(re, im)});
               } else {
                 h[g].x += re;
                 h[g].y += im;
               }
-              if (v.re != 0.f || v.im != 0.f) any = true;
+              if (v.re != 0 || v.im != 0) any = true;
             }
     }
-    float2* Sr = S + (size_t)rhs * fe;
+    V* Sr = reinterpret_cast<V*>(at(S, (size_t)rhs * fe));
     if (!sparse) {
-      HIPCHK(hipMemcpy(Sr, h.data(), fe * sizeof(float2), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(Sr, h.data(), fe * sizeof(V), hipMemcpyHostToDevice));
     } else {
       // overlapping patches add in patch order, from zero, as in the dense field
       std::stable_sort(ent.begin(), ent.end(), [](const Entry& a, const Entry& b) { return a.g < b.g; });
       std::vector<unsigned long long> hi;
-      std::vector<float2> hv;
+      std::vector<V> hv;
       for (size_t i = 0; i < ent.size();) {
-        float2 acc = make_float2(0.f, 0.f);
+        V acc = mkv_host<V>(R(0), R(0));
         const size_t g = ent[i].g;
         for (; i < ent.size() && ent[i].g == g; ++i) {
           acc.x += ent[i].v.x;
@@ -1280,12 +1360,12 @@ struct Solver {
         hi.push_back(g);
         hv.push_back(acc);
       }
-      HIPCHK(hipMemset(Sr, 0, fe * sizeof(float2)));
+      HIPCHK(hipMemset(Sr, 0, fe * sizeof(V)));
       if (!hi.empty()) {
         struct Tmp {  // device staging freed on every path out
           Solver* s;
           unsigned long long* i = nullptr;
-          float2* v = nullptr;
+          V* v = nullptr;
           ~Tmp() {
             s->dfree(i);
             s->dfree(v);
@@ -1294,10 +1374,10 @@ struct Solver {
         dalloc(&t.i, hi.size());
         dalloc(&t.v, hv.size());
         HIPCHK(hipMemcpy(t.i, hi.data(), hi.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(t.v, hv.data(), hv.size() * sizeof(float2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.v, hv.data(), hv.size() * sizeof(V), hipMemcpyHostToDevice));
         const long long m = (long long)hi.size();
         const unsigned blocks = (unsigned)std::min<long long>((m + 255) / 256, 4096);
-        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, 0, Sr, t.i, t.v, m);
+        hipLaunchKernelGGL(scatter_kernel<V>, dim3(blocks), dim3(256), 0, 0, Sr, t.i, t.v, m);
         HIPCHK(hipGetLastError());
         HIPCHK(hipDeviceSynchronize());
       }
@@ -1308,7 +1388,7 @@ struct Solver {
       const size_t nl = (size_t)(nvox / N[axes[0]]);
       std::vector<unsigned char> fl(nl, 0);
       for (int p = 0; p < np; ++p) {
-        const wsim_source_patch& sp = patches[p];
+        const Patch& sp = patches[p];
         long long pos[4], sz[4];
         for (int c = 0; c < 4; ++c) {
           pos[c] = (long long)std::llround(sp.pos[c]) - 1;
@@ -1330,6 +1410,7 @@ struct Solver {
   // warm start of one rhs: W0 = gamma * E0 (E0 unprimed, [pol][z][x][y])
   void set_warm(int rhs, const wsim_cfloat* E0, bool on_device) {
     require(rhs >= 0 && rhs < nrhs && E0 != nullptr, "bad argument");
+    require(!f64, "warm starts are built for complex64 solvers", WSIM_ERR_UNSUPPORTED);
     const size_t f1 = (size_t)field_elems();
     if (!W0) {
       dalloc(&W0, f1 * nrhs);
@@ -1344,7 +1425,7 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(dst, E0, f1 * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                           stream));
     const unsigned blocks = (unsigned)std::min<size_t>((f1 + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(gamma_mul_kernel, dim3(blocks), dim3(256), 0, stream, dst, gamma, (long long)nvox, C);
+    hipLaunchKernelGGL(gamma_mul_kernel<float2>, dim3(blocks), dim3(256), 0, stream, dst, gamma, (long long)nvox, C);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
     has_warm[rhs] = 1;
@@ -1435,12 +1516,12 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(d_niter, zero.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_last, inf.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, stream));
     const size_t fe = (size_t)field_elems() * nrhs;
-    HIPCHK(hipMemsetAsync(dE, 0, fe * sizeof(float2), stream));
+    HIPCHK(hipMemsetAsync(dE, 0, fe * cz(), stream));
     if (warm_mode) {
       // E' starts at gamma E0 (W0 is zero for rhs without a warm start)
       HIPCHK(hipMemcpyAsync(E, W0, fe * sizeof(float2), hipMemcpyDeviceToDevice, stream));
     } else {
-      HIPCHK(hipMemsetAsync(E, 0, fe * sizeof(float2), stream));
+      HIPCHK(hipMemsetAsync(E, 0, fe * cz(), stream));
     }
     // fused iterations: the first forward pass of iteration 1 runs here
     if (pre.fn) launch(pre);
@@ -1535,8 +1616,12 @@ struct Solver {
   void finish_e() {
     const long long fe = field_elems();
     const unsigned bx = (unsigned)std::min<long long>((fe + 255) / 256, 4096);
-    hipLaunchKernelGGL(finish_e_kernel, dim3(bx, (unsigned)nrhs), dim3(256), 0, stream, (const DevState*)dst, E,
-                       (const float2*)dE, fe);
+    if (f64)
+      hipLaunchKernelGGL(finish_e_kernel<double2>, dim3(bx, (unsigned)nrhs), dim3(256), 0, stream,
+                         (const DevState*)dst, reinterpret_cast<double2*>(E), reinterpret_cast<const double2*>(dE), fe);
+    else
+      hipLaunchKernelGGL(finish_e_kernel<float2>, dim3(bx, (unsigned)nrhs), dim3(256), 0, stream,
+                         (const DevState*)dst, E, (const float2*)dE, fe);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -1557,17 +1642,18 @@ struct Solver {
       HIPCHK(hipMemcpy(eall, d_eall + (size_t)rhs * hist_cap, m * sizeof(double), hipMemcpyDeviceToHost));
   }
 
-  void get_field(int rhs, wsim_cfloat* out) {
+  // out / in: complex values of the solver's precision
+  void get_field(int rhs, void* out) {
     require(rhs >= 0 && rhs < nrhs && out != nullptr, "bad argument");
     const size_t fe = (size_t)field_elems();
-    HIPCHK(hipMemcpy(out, E + (size_t)rhs * fe, fe * sizeof(float2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, at(E, (size_t)rhs * fe), fe * cz(), hipMemcpyDeviceToHost));
   }
 
-  void propagate(int w, const wsim_cfloat* in, wsim_cfloat* out) {
+  void propagate(int w, const void* in, void* out) {
     require(w >= 0 && w < nw && in && out, "bad argument");
     const size_t fe = (size_t)field_elems();
-    if (!probe) dalloc(&probe, fe);
-    HIPCHK(hipMemcpy(probe, in, fe * sizeof(float2), hipMemcpyHostToDevice));
+    if (!probe) dalloc_c(&probe, fe);
+    HIPCHK(hipMemcpy(probe, in, fe * cz(), hipMemcpyHostToDevice));
     HIPCHK(hipDeviceSynchronize());
     DevState ps{};
     ps.it = w + 1 + nw;  // wiggle w, past the source iterations
@@ -1581,7 +1667,7 @@ struct Solver {
     }
     for (const auto& L : probe_iter) launch(L);
     HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(out, probe, fe * sizeof(float2), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, probe, fe * cz(), hipMemcpyDeviceToHost));
   }
 
   int profile(int reps, int maxk, double* ms, double* bytes, const char** names, int* nk) {
@@ -1918,6 +2004,71 @@ int wsim_solver_create(wsim_solver** out, const wsim_solver_desc* desc, const ws
   });
 }
 
+int wsim_solver_create_f64(wsim_solver** out, const wsim_solver_desc_f64* desc, const wsim_cdouble* gamma,
+                           const wsim_wiggle_f64* wiggles) {
+  return guard([&]() -> int {
+    require(out != nullptr, "null output handle");
+    *out = nullptr;
+    require(desc != nullptr && gamma != nullptr && wiggles != nullptr, "null argument");
+    wsim_solver_desc d{};
+    for (int c = 0; c < 4; ++c) {
+      d.N[c] = desc->N[c];
+      d.roi[0][c] = desc->roi[0][c];
+      d.roi[1][c] = desc->roi[1][c];
+    }
+    d.nrhs = desc->nrhs;
+    d.nwiggles = desc->nwiggles;
+    d.epsilon = (float)desc->epsilon;
+    d.k02e = wsim_cfloat{(float)desc->k02e.re, (float)desc->k02e.im};
+    d.energy_interval = desc->energy_interval;
+    auto* h = new wsim_solver();
+    try {
+      h->s.f64 = true;
+      h->s.epsilon_d = desc->epsilon;
+      h->s.k02e_d = make_double2(desc->k02e.re, desc->k02e.im);
+      h->s.create(&d, gamma, wiggles);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_set_source_f64(wsim_solver* s, int rhs, int np, const wsim_source_patch_f64* p) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    s->s.set_source_f64(rhs, np, p);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_get_field_f64(wsim_solver* s, int rhs, wsim_cdouble* E_out) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    require(s->s.f64, "this solver computes in complex64: read its field with wsim_solver_get_field");
+    s->s.get_field(rhs, E_out);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_propagate_f64(wsim_solver* s, int w, const wsim_cdouble* in, wsim_cdouble* out) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    require(s->s.f64, "this solver computes in complex64: use wsim_solver_propagate");
+    s->s.propagate(w, in, out);
+    return WSIM_OK;
+  });
+}
+
+int wsim_solver_element_size(wsim_solver* s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    return (int)s->s.cz();
+  });
+}
+
 int wsim_solver_set_source(wsim_solver* s, int rhs, int np, const wsim_source_patch* p) {
   return guard([&]() -> int {
     require(s != nullptr, "null solver");
@@ -1946,6 +2097,7 @@ int wsim_solver_result(wsim_solver* s, int rhs, int64_t* n_iter, int* conv, doub
 int wsim_solver_get_field(wsim_solver* s, int rhs, wsim_cfloat* E_out) {
   return guard([&]() -> int {
     require(s != nullptr, "null solver");
+    require(!s->s.f64, "this solver computes in complex128: read its field with wsim_solver_get_field_f64");
     s->s.get_field(rhs, E_out);
     return WSIM_OK;
   });
@@ -1958,11 +2110,11 @@ int wsim_solver_device_ptr(wsim_solver* s, int which, int rhs, void** dptr) {
     require(rhs >= 0 && rhs < v.nrhs, "rhs out of range");
     const size_t off = (size_t)rhs * v.field_elems();
     switch (which) {
-      case 0: *dptr = v.E + off; break;
-      case 1: *dptr = v.dE + off; break;
+      case 0: *dptr = v.at(v.E, off); break;
+      case 1: *dptr = v.at(v.dE, off); break;
       case 2: *dptr = v.gamma; break;
-      case 3: *dptr = v.T + off; break;
-      case 4: *dptr = v.S + off; break;
+      case 3: *dptr = v.at(v.T, off); break;
+      case 4: *dptr = v.at(v.S, off); break;
       default: throw Error(WSIM_ERR_ARG, "which must be 0..4");
     }
     return WSIM_OK;
@@ -1972,6 +2124,7 @@ int wsim_solver_device_ptr(wsim_solver* s, int which, int rhs, void** dptr) {
 int wsim_solver_propagate(wsim_solver* s, int w, const wsim_cfloat* in, wsim_cfloat* out) {
   return guard([&]() -> int {
     require(s != nullptr, "null solver");
+    require(!s->s.f64, "this solver computes in complex128: use wsim_solver_propagate_f64");
     s->s.propagate(w, in, out);
     return WSIM_OK;
   });

@@ -132,6 +132,12 @@ struct PassArgs {
   float2 k02e;
   float2 inv_k02e;
   float invN;
+  // complex128 solves (Simulation.m:221-252 with single_precision = false):
+  // the same constants in double; every field, table and twiddle pointer
+  // above then addresses double2 / double data (typed float2 / float here)
+  double2 k02e_d;
+  double2 inv_k02e_d;
+  double invN_d;
   int roi_lo[4];  // 0-based inclusive, [y x z pol]
   int roi_hi[4];
   // one launch may carry several problems (entries of an array of PassArgs):
@@ -167,16 +173,27 @@ __host__ __device__ constexpr int its_per_stage(int ept, int C, int R) {
   return ept / (C * R) < 1 ? 1 : (ept / (C * R) < kMaxIts ? ept / (C * R) : kMaxIts);
 }
 
-struct ColInfo {
-  uint32_t off;  // element offset of row 0 from the workgroup's base (same for every component)
-  float2 rc;     // product of the other axes' ramps
-  float2 rc2;    // the same for the next iteration's wiggle (EPI_MIX_FWD)
-  float p[3];    // other axes' coordinates (axis entry unused)
-  float q;       // sum of squares of the other axes' coordinates
+// F = float (complex64 fields) or double (complex128 fields)
+template <class F>
+struct ColInfoT {
+  uint32_t off;    // element offset of row 0 from the workgroup's base (same for every component)
+  vec_of<F> rc;    // product of the other axes' ramps
+  vec_of<F> rc2;   // the same for the next iteration's wiggle (EPI_MIX_FWD)
+  F p[3];          // other axes' coordinates (axis entry unused)
+  F q;             // sum of squares of the other axes' coordinates
   int valid;
-  int roi_ok;    // other axes inside the ROI
-  int src;       // PSTD: the line carries source entries
+  int roi_ok;      // other axes inside the ROI
+  int src;         // PSTD: the line carries source entries
 };
+using ColInfo = ColInfoT<float>;
+// the host sizes the LDS column table with the same layout
+static_assert(sizeof(ColInfoT<float>) == 56 && sizeof(ColInfoT<double>) == 96, "column table layout");
+
+// the fields and tables of PassArgs as the kernel's element type
+template <class T>
+__device__ __forceinline__ const T* as(const void* p) { return reinterpret_cast<const T*>(p); }
+template <class T>
+__device__ __forceinline__ T* as_mut(void* p) { return reinterpret_cast<T*>(p); }
 
 // 32-bit element offsets from a workgroup-uniform base pointer, so the
 // compiler can use SGPR-base + 32-bit VGPR-offset addressing
@@ -197,6 +214,17 @@ __device__ __forceinline__ void stg(float2* base, uint32_t i, float2 v) {
   const v2f x = {v.x, v.y};
   *reinterpret_cast<__attribute__((address_space(1))) v2f*>((gptr)base + (size_t)(i * 8u)) = x;
 }
+// complex128: 16-byte elements (i * 16 stays below 2^32: a component holds
+// at most 2^28 voxels in double)
+using v2d = double __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ldg(const double2* base, uint32_t i) {
+  const v2d x = *reinterpret_cast<const __attribute__((address_space(1))) v2d*>((gptr_c)base + (size_t)(i * 16u));
+  return make_double2(x.x, x.y);
+}
+__device__ __forceinline__ void stg(double2* base, uint32_t i, double2 v) {
+  const v2d x = {v.x, v.y};
+  *reinterpret_cast<__attribute__((address_space(1))) v2d*>((gptr)base + (size_t)(i * 16u)) = x;
+}
 // A pointer the whole workgroup shares, moved to scalar registers.  Values
 // computed with 64-bit multiplies land in vector registers even when every
 // lane holds the same one, and a vector base costs each access a 64-bit
@@ -214,6 +242,10 @@ __device__ __forceinline__ float ldt(const float* p) {
   return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)p);
 }
 __device__ __forceinline__ float2 ldt(const float2* p) { return ldg(p, 0); }
+__device__ __forceinline__ double ldt(const double* p) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) double*>((gptr_c)p);
+}
+__device__ __forceinline__ double2 ldt(const double2* p) { return ldg(p, 0); }
 __device__ __forceinline__ float ldf(const float* base, uint32_t i) {
   return *reinterpret_cast<const __attribute__((address_space(1))) float*>((gptr_c)base + (size_t)(i * 4u));
 }
@@ -235,6 +267,35 @@ __device__ __forceinline__ void stl(lf2 p, int i, float2 v) {
   const v2f x = {v.x, v.y};
   p[i] = x;
 }
+using ld2 = WSIM_LDS v2d*;
+using lcd2 = const WSIM_LDS v2d*;
+using lcd = const WSIM_LDS double*;
+__device__ __forceinline__ double2 ldl(lcd2 p, int i) {
+  const v2d x = p[i];
+  return make_double2(x.x, x.y);
+}
+__device__ __forceinline__ void stl(ld2 p, int i, double2 v) {
+  const v2d x = {v.x, v.y};
+  p[i] = x;
+}
+// LDS pointer types by scalar type
+template <class F> struct LdsOf;
+template <> struct LdsOf<float> {
+  using vec = lf2;
+  using cvec = lcf2;
+  using cscal = lcf;
+  using scal = WSIM_LDS float*;
+};
+template <> struct LdsOf<double> {
+  using vec = ld2;
+  using cvec = lcd2;
+  using cscal = lcd;
+  using scal = WSIM_LDS double*;
+};
+// reciprocal of the Green's-function denominator: the hardware approximation
+// in single precision (1 ulp), a correctly rounded division in double
+__device__ __forceinline__ float rcp_green(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_green(double x) { return 1.0 / x; }
 
 #ifndef WSIM_PAD_SHIFT  // contiguous lines in LDS: one padding slot every 2^WSIM_PAD_SHIFT rows
 #define WSIM_PAD_SHIFT 3
@@ -247,38 +308,45 @@ __device__ __forceinline__ void stl(lf2 p, int i, float2 v) {
 #define WSIM_FUSED_CHUNK 4
 #endif
 
-template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
+// F: the scalar type of the field (float: complex64, double: complex128)
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float>
 struct Pass {
+  using V = vec_of<F>;
+  using CI = ColInfoT<F>;
+  using L2 = typename LdsOf<F>::vec;
+  using LC2 = typename LdsOf<F>::cvec;
+  using LCS = typename LdsOf<F>::cscal;
+  static constexpr bool F64 = sizeof(F) == 8;
   const PassArgs& a;
-  lf2 lds;
-  const ColInfo* cols;
+  L2 lds;
+  const CI* cols;
   int tid;
   bool add_src;
   bool warm_step;  // one of the first Nw steps of a warm start
-  float inv_nw;
+  F inv_nw;
   uint32_t lg_ncol;
-  lcf2 ramp_a;  // this wiggle's ramp along the axis
-  lcf2 ramp2_a; // the next iteration's (EPI_MIX_FWD)
+  LC2 ramp_a;  // this wiggle's ramp along the axis
+  LC2 ramp2_a; // the next iteration's (EPI_MIX_FWD)
   bool add_src2;         // the next iteration adds the source (EPI_MIX_FWD)
   bool warm_step2;       // the next iteration adds the warm-start portion
-  lcf pe_a;
-  const float2* in_b[C];  // workgroup bases, per component
-  float2* out_b[C];
-  const float2* S_b[C];
-  const float2* W0_b[C];
-  float2* dE_b[C];
-  float2* E_b[C];
-  const float2* gam_b;
+  LCS pe_a;
+  const V* in_b[C];  // workgroup bases, per component
+  V* out_b[C];
+  const V* S_b[C];
+  const V* W0_b[C];
+  V* dE_b[C];
+  V* E_b[C];
+  const V* gam_b;
   // PSTD epilogue: current and previous time level, coefficients, source
-  const float2* fcur_b[C];
-  float2* fprv_b[C];
-  const float2* Sp_b[C];
+  const V* fcur_b[C];
+  V* fprv_b[C];
+  const V* Sp_b[C];
   const float* c1_b;
   const float* c2_b;
   const float* c3_b;
-  float2 amp, phase;
+  V amp, phase;
   uint32_t nt;   // threads of the workgroup (NT is the compile-time maximum)
-  ColInfo cfix;  // strided passes: the thread's column never changes
+  CI cfix;  // strided passes: the thread's column never changes
   double eacc;
   // E' += dE' is carried out on even iterations only, as E' += dE'_{k-1} +
   // dE'_k: the mix reads dE'_{k-1} and forms dE'_k anyway, so E' is read and
@@ -328,7 +396,18 @@ struct Pass {
     }
   }
 
-  __device__ __forceinline__ const ColInfo& col(int t) const { return CONTIG ? cols[t] : cfix; }
+  __device__ __forceinline__ const CI& col(int t) const { return CONTIG ? cols[t] : cfix; }
+
+  // the problem's constants in the field's precision
+  __device__ __forceinline__ V k02e() const {
+    if constexpr (F64) return a.k02e_d; else return a.k02e;
+  }
+  __device__ __forceinline__ V inv_k02e() const {
+    if constexpr (F64) return a.inv_k02e_d; else return a.inv_k02e;
+  }
+  __device__ __forceinline__ F invN() const {
+    if constexpr (F64) return a.invN_d; else return a.invN;
+  }
 
   // The butterflies of a stage that this thread takes are b = tid + it*nt,
   // it = 0, 1, ...  Rather than dividing every b, a cursor starts from one
@@ -380,15 +459,15 @@ struct Pass {
     if (c.k >= s.Ns) c.k -= s.Ns;
   }
   // HBM index of row j of the cursor's column
-  __device__ __forceinline__ uint32_t gbase(const Cursor& c, const ColInfo& ci) const {
+  __device__ __forceinline__ uint32_t gbase(const Cursor& c, const CI& ci) const {
     return CONTIG ? ci.off + (uint32_t)c.j : c.g;
   }
 
   // ---- element transforms --------------------------------------------------
   // first-stage operand before the ramps: dE' (+ W0/Nw in the warm steps,
   // + i/eps/Nw * S in the source steps)
-  __device__ __forceinline__ float2 load_raw(uint32_t i, int c) const {
-    float2 v = ldg(in_b[c], i);
+  __device__ __forceinline__ V load_raw(uint32_t i, int c) const {
+    V v = ldg(in_b[c], i);
     if (PRO == PRO_SRC_RAMP) {
       if (warm_step) v = cadd(v, cscale(ldg(W0_b[c], i), inv_nw));
       if (add_src) v = cadd(v, ldg(S_b[c], i));
@@ -397,12 +476,12 @@ struct Pass {
   }
 
   // scalar Green's function times 1/N (f_g0_scalar.m:6)
-  __device__ __forceinline__ float2 green_scalar(const ColInfo& ci, int row, float2 v) const {
-    const float pa = pe_a[row];
-    const float dr = ci.q + (pa * pa - a.k02e.x);
-    const float di = -a.k02e.y;
-    const float s = a.invN * __builtin_amdgcn_rcpf(dr * dr + di * di);
-    return cmul(v, make_float2(dr * s, -di * s));
+  __device__ __forceinline__ V green_scalar(const CI& ci, int row, V v) const {
+    const F pa = pe_a[row];
+    const F dr = ci.q + (pa * pa - k02e().x);
+    const F di = -k02e().y;
+    const F s = invN() * rcp_green(dr * dr + di * di);
+    return cmul(v, mkv<V>(dr * s, -di * s));
   }
 
   template <int R>
@@ -412,7 +491,7 @@ struct Pass {
   // first stage from HBM.  Only loads here: a use of a loaded value inside
   // the guarded block would make every iteration wait for its own loads.
   template <int R>
-  __device__ __forceinline__ void stage_load_global(float2* v, const Stage& s) const {
+  __device__ __forceinline__ void stage_load_global(V* v, const Stage& s) const {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
@@ -420,14 +499,14 @@ struct Pass {
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
       const bool lv = live(cu, s);
-      const ColInfo& ci = col(lv ? cu.t : 0);
+      const CI& ci = col(lv ? cu.t : 0);
       const bool ok = lv && ci.valid;
       const uint32_t g0 = gbase(cu, ci);
 #pragma unroll
       for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          v[(c * ITS + it) * R + r] = ok ? load_raw(g0 + (uint32_t)r * gs, c) : make_float2(0.f, 0.f);
+          v[(c * ITS + it) * R + r] = ok ? load_raw(g0 + (uint32_t)r * gs, c) : mkv<V>(F(0), F(0));
       step(cu, w, s);
     }
     if (PRO == PRO_SRC_RAMP) {
@@ -436,7 +515,7 @@ struct Pass {
 #pragma unroll
       for (int it = 0; it < ITS; ++it) {
         if (live(cu, s)) {
-          const float2 rc = col(cu.t).rc;
+          const V rc = col(cu.t).rc;
 #pragma unroll
           for (int c = 0; c < C; ++c)
 #pragma unroll
@@ -454,7 +533,7 @@ struct Pass {
   // a butterfly issue together.  Waves with no live butterfly (a stage with
   // fewer butterflies than threads) skip the reads.
   template <int R, bool FIRST = false>
-  __device__ __forceinline__ void stage_load_lds(float2* v, const Stage& s, lcf2 tw) const {
+  __device__ __forceinline__ void stage_load_lds(V* v, const Stage& s, LC2 tw) const {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
@@ -463,33 +542,33 @@ struct Pass {
       const bool lv = live(cu, s);
       if (__builtin_amdgcn_ballot_w64(lv) != 0) {
         const int t = lv ? cu.t : 0, j = lv ? cu.j : 0, k = lv ? cu.k : 0;
-        lcf2 twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
+        LC2 twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const float2 x = ldl(lds, lrow(t, c, j, s.nb, s.nbp, r));
+            const V x = ldl(lds, lrow(t, c, j, s.nb, s.nbp, r));
             v[(c * ITS + it) * R + r] = (r == 0 || FIRST) ? x : cmul(x, ldl(twk, (r - 1) * s.Ns));
           }
       } else {
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int r = 0; r < R; ++r) v[(c * ITS + it) * R + r] = make_float2(0.f, 0.f);
+          for (int r = 0; r < R; ++r) v[(c * ITS + it) * R + r] = mkv<V>(F(0), F(0));
       }
       step(cu, w, s);
     }
   }
 
   template <int R, int DIR>
-  __device__ __forceinline__ void stage_compute(float2* v) const {
+  __device__ __forceinline__ void stage_compute(V* v) const {
     constexpr int ITS = its<R>();
 #pragma unroll
     for (int k = 0; k < C * ITS; ++k) dft<R, DIR>(v + k * R);
   }
 
   template <int R>
-  __device__ __forceinline__ void stage_store_lds(const float2* v, const Stage& s) const {
+  __device__ __forceinline__ void stage_store_lds(const V* v, const Stage& s) const {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
@@ -510,7 +589,7 @@ struct Pass {
   // epilogue applied on the way out (EPI, or EPI_NONE for the forward tail of
   // EPI_MIX_FWD)
   template <int R, int E2 = EPI>
-  __device__ __forceinline__ void stage_store_global(const float2* v, const Stage& s) {
+  __device__ __forceinline__ void stage_store_global(const V* v, const Stage& s) {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
@@ -519,13 +598,13 @@ struct Pass {
     for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
       if (live(cu, s)) {
         const int t = cu.t, j = cu.j;
-        const ColInfo& ci = col(t);
+        const CI& ci = col(t);
         if (!ci.valid) continue;
         if (E2 == EPI_MIX || E2 == EPI_MIX_FWD) {
           mix_rows<R>(v, it, t, j, s.nb, ci);
           continue;
         }
-        if (E2 == EPI_PSTD) {
+        if constexpr (E2 == EPI_PSTD) {
           pstd_rows<R>(v, it, j, s.nb, ci);
           continue;
         }
@@ -538,7 +617,7 @@ struct Pass {
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, v[(c * ITS + it) * R + r]);
           } else {  // EPI_RAMP: undo the ramps only (propagation probe)
-            const float2 ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
+            const V ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
 #pragma unroll
             for (int c = 0; c < C; ++c) stg(out_b[c], i, cmul(v[(c * ITS + it) * R + r], ramp));
           }
@@ -553,7 +632,7 @@ struct Pass {
   // a load after a store to a possibly aliasing address would wait for the
   // previous row's whole load-compute-store chain.
   template <int R>
-  __device__ __forceinline__ void mix_rows(const float2* v, int it, int t, int j, int nb, const ColInfo& ci) {
+  __device__ __forceinline__ void mix_rows(const V* v, int it, int t, int j, int nb, const CI& ci) {
     constexpr int ITS = its<R>();
     // rows per chunk: 4 for scalar fields (118 VGPRs in the fused variant,
     // four waves per SIMD; 2 rows ran the pass 4% slower), 1 per component
@@ -563,7 +642,7 @@ struct Pass {
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += CH) {
-      float2 gm[CH], d0[CH][C], e0[CH][C], w0[CH][C];
+      V gm[CH], d0[CH][C], e0[CH][C], w0[CH][C];
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
         if (r0 + q >= R) break;
@@ -572,8 +651,8 @@ struct Pass {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           d0[q][c] = ldg(dE_b[c], i);
-          e0[q][c] = acc_e ? ldg(E_b[c], i) : make_float2(0.f, 0.f);
-          w0[q][c] = warm_step ? cscale(ldg(W0_b[c], i), inv_nw) : make_float2(0.f, 0.f);
+          e0[q][c] = acc_e ? ldg(E_b[c], i) : mkv<V>(F(0), F(0));
+          w0[q][c] = warm_step ? cscale(ldg(W0_b[c], i), inv_nw) : mkv<V>(F(0), F(0));
         }
       }
 #pragma unroll
@@ -582,18 +661,18 @@ struct Pass {
         const int r = r0 + q;
         const int row = j + r * nb;
         const uint32_t i = gidx(ci.off, row);
-        const float2 ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
-        const float2 c1 = make_float2(1.f - 0.7f * gm[q].x, -0.7f * gm[q].y);
-        const float2 g2 = cmul(gm[q], gm[q]);
-        const float2 c2 = make_float2(-0.7f * g2.y, 0.7f * g2.x);
+        const V ramp = cconj(cmul(ldl(ramp_a, row), ci.rc));
+        const V c1 = mkv<V>(F(1) - F(0.7) * gm[q].x, F(-0.7) * gm[q].y);
+        const V g2 = cmul(gm[q], gm[q]);
+        const V c2 = mkv<V>(F(-0.7) * g2.y, F(0.7) * g2.x);
         const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float2 ep = cmul(v[(c * ITS + it) * R + r], ramp);
+          const V ep = cmul(v[(c * ITS + it) * R + r], ramp);
           // warm start (first Nw steps): the increment of the correction
           // W0/Nw, i.e. (M_w - 1) W0/Nw, joins the recurrence
-          const float2 w = w0[q][c];
-          const float2 dn = csub(csub(cmul(c1, cadd(d0[q][c], w)), cmul(c2, ep)), w);
+          const V w = w0[q][c];
+          const V dn = csub(csub(cmul(c1, cadd(d0[q][c], w)), cmul(c2, ep)), w);
           stg(dE_b[c], i, dn);
           if (acc_e) stg(E_b[c], i, cadd(cadd(e0[q][c], d0[q][c]), dn));
           if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3])
@@ -603,7 +682,7 @@ struct Pass {
             // ramp step of the first forward pass would form it
             // (the source is read here, not with the other operands: it is
             // needed in the first Nw iterations only and would hold registers)
-            float2 x = dn;
+            V x = dn;
             if (warm_step2) x = cadd(x, w0[q][c]);
             if (add_src2) x = cadd(x, ldg(S_b[c], i));
             stl(lds, lidx(t, c, row), cmul(x, cmul(ldl(ramp2_a, row), ci.rc2)));
@@ -618,14 +697,15 @@ struct Pass {
   // into E_prev's buffer; energy partials of E_next - E exp(i dphi) in the ROI.
   // Operands of a chunk of rows are loaded before any store (as mix_rows).
   template <int R>
-  __device__ __forceinline__ void pstd_rows(const float2* v, int it, int j, int nb, const ColInfo& ci) {
+  __device__ __forceinline__ void pstd_rows(const V* v, int it, int j, int nb, const CI& ci) {
+    static_assert(!F64, "the PSTD step is built for complex64 fields only");
     constexpr int ITS = its<R>();
     constexpr int CH = 4 < R ? 4 : R;
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += CH) {
       float k1[CH], k2[CH], k3[CH];
-      float2 ec[CH][C], ep[CH][C], sv[CH][C];
+      V ec[CH][C], ep[CH][C], sv[CH][C];
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
         if (r0 + q >= R) break;
@@ -637,7 +717,7 @@ struct Pass {
         for (int c = 0; c < C; ++c) {
           ec[q][c] = ldg(fcur_b[c], i);
           ep[q][c] = ldg(fprv_b[c], i);
-          sv[q][c] = ci.src ? ldg(Sp_b[c], i) : make_float2(0.f, 0.f);
+          sv[q][c] = ci.src ? ldg(Sp_b[c], i) : mkv<V>(F(0), F(0));
         }
       }
 #pragma unroll
@@ -648,14 +728,14 @@ struct Pass {
         const bool in_roi = ci.roi_ok && row >= rlo && row <= rhi;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          float2 et = v[(c * ITS + it) * R + r0 + q];
+          V et = v[(c * ITS + it) * R + r0 + q];
           if (ci.src) et = cadd(et, cmul(amp, sv[q][c]));
-          const float2 e = ec[q][c], em = ep[q][c];
-          const float2 en = make_float2((k2[q] * e.x + k1[q] * em.x) + k3[q] * et.x,
+          const V e = ec[q][c], em = ep[q][c];
+          const V en = mkv<V>((k2[q] * e.x + k1[q] * em.x) + k3[q] * et.x,
                                         (k2[q] * e.y + k1[q] * em.y) + k3[q] * et.y);
           stg(fprv_b[c], i, en);
           if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3]) {
-            const float2 d = csub(en, cmul(e, phase));
+            const V d = csub(en, cmul(e, phase));
             eacc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
           }
         }
@@ -665,7 +745,7 @@ struct Pass {
 
   // Green's function on the forward last-stage outputs (rows j + r*nb)
   template <int R>
-  __device__ __forceinline__ void stage_green(float2* v, const Stage& s) const {
+  __device__ __forceinline__ void stage_green(V* v, const Stage& s) const {
     constexpr int ITS = its<R>();
     const Walk w = walk(s);
     Cursor cu = start(s);
@@ -673,33 +753,33 @@ struct Pass {
     for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
       if (live(cu, s)) {
         const int j = cu.j;
-        const ColInfo& ci = col(cu.t);
+        const CI& ci = col(cu.t);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int row = j + r * s.nb;
           if (C == 1) {
             if (a.pstd) {
               // PSTD Laplacian -|p|^2 times 1/N (PSTD.m:68, 108)
-              const float pa = pe_a[row];
-              v[it * R + r] = cscale(v[it * R + r], -(ci.q + pa * pa) * a.invN);
+              const F pa = pe_a[row];
+              v[it * R + r] = cscale(v[it * R + r], -(ci.q + pa * pa) * invN());
             } else {
               v[it * R + r] = green_scalar(ci, row, v[it * R + r]);
             }
           } else {
             // f_g0_vector.m:4-8; component 0 <-> x <-> array axis 1
-            const float pr = pe_a[row];
-            const float py = a.axis == 0 ? pr : ci.p[0];
-            const float px = a.axis == 1 ? pr : ci.p[1];
-            const float pz = a.axis == 2 ? pr : ci.p[2];
-            const float2 fx = v[(0 * ITS + it) * R + r];
-            const float2 fy = v[(1 * ITS + it) * R + r];
-            const float2 fz = v[(2 * ITS + it) * R + r];
-            float2 dv = make_float2(px * fx.x + py * fy.x + pz * fz.x, px * fx.y + py * fy.y + pz * fz.y);
-            dv = cmul(dv, a.inv_k02e);
-            const float dr = px * px + py * py + pz * pz - a.k02e.x;
-            const float di = -a.k02e.y;
-            const float sc = a.invN * __builtin_amdgcn_rcpf(dr * dr + di * di);
-            const float2 g0 = make_float2(dr * sc, -di * sc);
+            const F pr = pe_a[row];
+            const F py = a.axis == 0 ? pr : ci.p[0];
+            const F px = a.axis == 1 ? pr : ci.p[1];
+            const F pz = a.axis == 2 ? pr : ci.p[2];
+            const V fx = v[(0 * ITS + it) * R + r];
+            const V fy = v[(1 * ITS + it) * R + r];
+            const V fz = v[(2 * ITS + it) * R + r];
+            V dv = mkv<V>(px * fx.x + py * fy.x + pz * fz.x, px * fx.y + py * fy.y + pz * fz.y);
+            dv = cmul(dv, inv_k02e());
+            const F dr = px * px + py * py + pz * pz - k02e().x;
+            const F di = -k02e().y;
+            const F sc = invN() * rcp_green(dr * dr + di * di);
+            const V g0 = mkv<V>(dr * sc, -di * sc);
             v[(0 * ITS + it) * R + r] = cmul(g0, csub(fx, cscale(dv, px)));
             v[(1 * ITS + it) * R + r] = cmul(g0, csub(fy, cscale(dv, py)));
             v[(2 * ITS + it) * R + r] = cmul(g0, csub(fz, cscale(dv, pz)));
@@ -732,8 +812,13 @@ struct Pass {
   }
 #endif
 
-template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT>
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float>
 __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restrict__ ap) {
+  using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT, F>;
+  using V = typename P::V;
+  using CI = typename P::CI;
+  using L2 = typename P::L2;
+  using LC2 = typename P::LC2;
   // the arguments live in device memory (read through the scalar cache):
   // indexing the stage table of a by-value kernel argument would copy the
   // whole struct to scratch
@@ -761,27 +846,27 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   const DevState* st = a.st;
   if (st->stopped[rhs]) return;
 
-  // LDS: [ColInfo x ncol][ramp_a x n][(ramp2 x n)][pe_a x n][twiddles][transform data]
+  // LDS: [CI x ncol][ramp_a x n][(ramp2 x n)][pe_a x n][twiddles][transform data]
   constexpr bool FUSE = EPI == EPI_MIX_FWD;
-  ColInfo* cols = reinterpret_cast<ColInfo*>(smem);
-  const size_t rn = ((size_t)a.n * sizeof(float2) + 15) & ~(size_t)15;
-  const size_t off_ramp = (a.ncol * sizeof(ColInfo) + 15) & ~(size_t)15;
+  CI* cols = reinterpret_cast<CI*>(smem);
+  const size_t rn = ((size_t)a.n * sizeof(V) + 15) & ~(size_t)15;
+  const size_t off_ramp = (a.ncol * sizeof(CI) + 15) & ~(size_t)15;
   const size_t off_ramp2 = off_ramp + rn;
   const size_t off_pe = off_ramp + (FUSE ? 2 : 1) * rn;
   const int ntf = (MODE == MODE_INV && !FUSE) ? 0 : a.ntwf;  // forward twiddles used by this pass
   const int nti = MODE == MODE_FWD ? 0 : a.ntwi;  // inverse twiddles used by this pass
-  const size_t off_twf = off_pe + (((size_t)a.n * sizeof(float) + 15) & ~(size_t)15);
-  const size_t off_twi = off_twf + (size_t)ntf * sizeof(float2);
-  const size_t off_data = off_twi + (size_t)nti * sizeof(float2);
-  lf2 ramp_l = (lf2)(smem + off_ramp);
-  lf2 ramp2_l = (lf2)(smem + off_ramp2);
-  WSIM_LDS float* pe_l = (WSIM_LDS float*)(smem + off_pe);
-  lf2 twf_l = (lf2)(smem + off_twf);
-  lf2 twi_l = (lf2)(smem + off_twi);
-  lf2 lds = (lf2)(smem + off_data);
+  const size_t off_twf = off_pe + (((size_t)a.n * sizeof(F) + 15) & ~(size_t)15);
+  const size_t off_twi = off_twf + (size_t)ntf * sizeof(V);
+  const size_t off_data = off_twi + (size_t)nti * sizeof(V);
+  L2 ramp_l = (L2)(smem + off_ramp);
+  L2 ramp2_l = (L2)(smem + off_ramp2);
+  typename LdsOf<F>::scal pe_l = (typename LdsOf<F>::scal)(smem + off_pe);
+  L2 twf_l = (L2)(smem + off_twf);
+  L2 twi_l = (L2)(smem + off_twi);
+  L2 lds = (L2)(smem + off_data);
   const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
-  for (int i = threadIdx.x; i < ntf; i += nthr) stl(twf_l, i, ldt(a.twf + i));
-  for (int i = threadIdx.x; i < nti; i += nthr) stl(twi_l, i, ldt(a.twi + i));
+  for (int i = threadIdx.x; i < ntf; i += nthr) stl(twf_l, i, ldt(as<V>(a.twf) + i));
+  for (int i = threadIdx.x; i < nti; i += nthr) stl(twi_l, i, ldt(as<V>(a.twi) + i));
 
   const int it_cur = st->it;
   const int nw = st->nw;
@@ -800,13 +885,13 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
 
   for (int r = threadIdx.x; r < a.n; r += nthr) {
-    stl(ramp_l, r, ldt(a.ramp[a.axis] + (size_t)w * a.n + r));
-    if (FUSE) stl(ramp2_l, r, ldt(a.ramp[a.axis] + (size_t)w2 * a.n + r));
-    pe_l[r] = ldt(a.pe[a.axis] + (size_t)w * a.n + r);
+    stl(ramp_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w * a.n + r));
+    if (FUSE) stl(ramp2_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w2 * a.n + r));
+    pe_l[r] = ldt(as<F>(a.pe[a.axis]) + (size_t)w * a.n + r);
   }
   // per-column coordinates (axes other than a.axis)
   for (int t = threadIdx.x; t < a.ncol; t += nthr) {
-    ColInfo ci;
+    CI ci;
     int crd[3] = {0, 0, 0};
     int valid;
     if (CONTIG) {
@@ -833,18 +918,18 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     ci.valid = valid;
     ci.src = 0;
     if (CONTIG && a.srcline != nullptr && valid) ci.src = a.srcline[bx * a.ncol + t];
-    float2 rc = make_float2(1.f, 0.f), rc2 = make_float2(1.f, 0.f);
-    float q = 0.f;
+    V rc = mkv<V>(F(1), F(0)), rc2 = mkv<V>(F(1), F(0));
+    F q = F(0);
     int roi_ok = 1;
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      ci.p[b] = 0.f;
+      ci.p[b] = F(0);
       if (b == a.axis) continue;
-      const float pb = ldt(a.pe[b] + (size_t)w * a.N[b] + crd[b]);
+      const F pb = ldt(as<F>(a.pe[b]) + (size_t)w * a.N[b] + crd[b]);
       ci.p[b] = pb;
       q += pb * pb;
-      rc = cmul(rc, ldt(a.ramp[b] + (size_t)w * a.N[b] + crd[b]));
-      if (FUSE) rc2 = cmul(rc2, ldt(a.ramp[b] + (size_t)w2 * a.N[b] + crd[b]));
+      rc = cmul(rc, ldt(as<V>(a.ramp[b]) + (size_t)w * a.N[b] + crd[b]));
+      if (FUSE) rc2 = cmul(rc2, ldt(as<V>(a.ramp[b]) + (size_t)w2 * a.N[b] + crd[b]));
       roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
     }
     ci.rc = rc;
@@ -855,9 +940,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
   __syncthreads();
 
-  using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT>;
   P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw,
-      st->warm != 0 && a.W0 != nullptr && it_cur <= nw, 1.f / (float)nw, 0u, ramp_l, ramp2_l,
+      st->warm != 0 && a.W0 != nullptr && it_cur <= nw, F(1) / F(nw), 0u, ramp_l, ramp2_l,
       FUSE && a.S != nullptr && it_cur + 1 <= nw, FUSE && st->warm != 0 && a.W0 != nullptr && it_cur + 1 <= nw,
       pe_l};
   p.eacc = 0.0;
@@ -869,8 +953,8 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
   const long long rbase = (long long)rhs * a.rhs_stride + vblk;
   // PSTD: the first pass transforms the current time level
-  const float2* in_sel = (PRO == PRO_SRC_RAMP && a.pstd) ? a.fld[(it_cur - 1) & 1] : a.in;
-  if (EPI == EPI_PSTD) {
+  const V* in_sel = as<V>((PRO == PRO_SRC_RAMP && a.pstd) ? a.fld[(it_cur - 1) & 1] : a.in);
+  if constexpr (EPI == EPI_PSTD) {
     p.c1_b = wave_uniform(a.coef + vblk);
     p.c2_b = wave_uniform(a.coef + a.nvox + vblk);
     p.c3_b = wave_uniform(a.coef + 2 * a.nvox + vblk);
@@ -880,25 +964,25 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const long long cb = rbase + (long long)c * a.nvox;
-    if (EPI == EPI_PSTD) {
+    if constexpr (EPI == EPI_PSTD) {
       p.fcur_b[c] = wave_uniform(a.fld[(it_cur - 1) & 1] + cb);
       p.fprv_b[c] = wave_uniform(a.fld[it_cur & 1] + cb);
       p.Sp_b[c] = wave_uniform(a.Sp + cb);
     }
     p.in_b[c] = wave_uniform(in_sel + cb);
-    p.out_b[c] = wave_uniform(a.out + cb);
-    p.S_b[c] = wave_uniform(a.S + cb);
-    p.W0_b[c] = wave_uniform(a.W0 + cb);
-    p.dE_b[c] = wave_uniform(a.dE + cb);
-    p.E_b[c] = wave_uniform(a.E + cb);
+    p.out_b[c] = wave_uniform(as_mut<V>(a.out) + cb);
+    p.S_b[c] = wave_uniform(as<V>(a.S) + cb);
+    p.W0_b[c] = wave_uniform(as<V>(a.W0) + cb);
+    p.dE_b[c] = wave_uniform(as_mut<V>(a.dE) + cb);
+    p.E_b[c] = wave_uniform(as_mut<V>(a.E) + cb);
   }
-  p.gam_b = wave_uniform(a.gamma + vblk);
+  p.gam_b = wave_uniform(as<V>(a.gamma) + vblk);
 
   const AxisPlan& pl = a.plan;
   const int S = pl.nst;
   constexpr bool MID = MODE == MODE_MID;
   const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
-  lcf2 tw = (MODE == MODE_INV) ? twi_l : twf_l;
+  LC2 tw = (MODE == MODE_INV) ? twi_l : twf_l;
   constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
 
   // Each region below is its own radix switch, so the register allocator
@@ -907,7 +991,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     // single stage: HBM -> registers -> HBM (the middle pass applies the
     // Green's function and the inverse DFT in registers)
     WSIM_RADIX_SWITCH(fs[0].R,
-      float2 v[P::template its<R>() * C * R];
+      V v[P::template its<R>() * C * R];
       p.template stage_load_global<R>(v, fs[0]);
       p.template stage_compute<R, D0>(v);
       if (MID) {
@@ -921,7 +1005,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   } else {
     // first stage: HBM -> registers -> LDS
     WSIM_RADIX_SWITCH(fs[0].R,
-      float2 v[P::template its<R>() * C * R];
+      V v[P::template its<R>() * C * R];
       p.template stage_load_global<R>(v, fs[0]);
       p.template stage_compute<R, D0>(v);
       p.template stage_store_lds<R>(v, fs[0]);
@@ -931,7 +1015,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     for (int s = 1; s < S - 1; ++s) {
       const Stage st_ = fs[s];
       WSIM_RADIX_SWITCH(st_.R,
-        float2 v[P::template its<R>() * C * R];
+        V v[P::template its<R>() * C * R];
         p.template stage_load_lds<R>(v, st_, tw);
         __syncthreads();
         p.template stage_compute<R, D0>(v);
@@ -943,7 +1027,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     if (!MID) {
       // last stage: LDS -> registers -> HBM
       WSIM_RADIX_SWITCH(sl.R,
-        float2 v[P::template its<R>() * C * R];
+        V v[P::template its<R>() * C * R];
         p.template stage_load_lds<R>(v, sl, tw);
         if (FUSE) __syncthreads();  // the epilogue writes the next pass's operand to LDS
         p.template stage_compute<R, D0>(v);
@@ -955,7 +1039,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
         __syncthreads();
         const Stage* ff = pl.f;
         WSIM_RADIX_SWITCH(ff[0].R,
-          float2 v[P::template its<R>() * C * R];
+          V v[P::template its<R>() * C * R];
           p.template stage_load_lds<R, true>(v, ff[0], twf_l);
           __syncthreads();
           p.template stage_compute<R, -1>(v);
@@ -965,7 +1049,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
         for (int s = 1; s < S - 1; ++s) {
           const Stage st_ = ff[s];
           WSIM_RADIX_SWITCH(st_.R,
-            float2 v[P::template its<R>() * C * R];
+            V v[P::template its<R>() * C * R];
             p.template stage_load_lds<R>(v, st_, twf_l);
             __syncthreads();
             p.template stage_compute<R, -1>(v);
@@ -975,7 +1059,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
         }
         const Stage fl = ff[S - 1];
         WSIM_RADIX_SWITCH(fl.R,
-          float2 v[P::template its<R>() * C * R];
+          V v[P::template its<R>() * C * R];
           p.template stage_load_lds<R>(v, fl, twf_l);
           p.template stage_compute<R, -1>(v);
           p.template stage_store_global<R, EPI_NONE>(v, fl);
@@ -985,7 +1069,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       // forward last stage, Green's function, inverse first stage (same
       // radix, Ns = 1, no twiddles), back to LDS
       WSIM_RADIX_SWITCH(sl.R,
-        float2 v[P::template its<R>() * C * R];
+        V v[P::template its<R>() * C * R];
         p.template stage_load_lds<R>(v, sl, tw);
         __syncthreads();
         p.template stage_compute<R, -1>(v);
@@ -997,7 +1081,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       for (int s = 1; s < S - 1; ++s) {
         const Stage st_ = pl.i[s];
         WSIM_RADIX_SWITCH(st_.R,
-          float2 v[P::template its<R>() * C * R];
+          V v[P::template its<R>() * C * R];
           p.template stage_load_lds<R>(v, st_, twi_l);
           __syncthreads();
           p.template stage_compute<R, 1>(v);
@@ -1007,7 +1091,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
       }
       const Stage il = pl.i[S - 1];
       WSIM_RADIX_SWITCH(il.R,
-        float2 v[P::template its<R>() * C * R];
+        V v[P::template its<R>() * C * R];
         p.template stage_load_lds<R>(v, il, twi_l);
         p.template stage_compute<R, 1>(v);
         p.template stage_store_global<R>(v, il);

@@ -223,8 +223,11 @@ class WaveSimBase:
         opt = dict(options)
         if "wiggle" in opt:
             opt["ACC"] = opt["wiggle"]
-        if not opt.get("single_precision", True):
-            raise NotImplementedError("the GPU solver computes in complex64 (single_precision=true)")
+        # single_precision (Simulation.m:31): complex64 fields (default) or
+        # complex128 (data_array, Simulation.m:221-252)
+        self.single = bool(opt.get("single_precision", True))
+        self.cdtype = np.complex64 if self.single else np.complex128
+        self.rdtype = np.float32 if self.single else np.float64
         self.opt = opt
         self.lam = float(opt.get("lambda", 1.0))
         self.callback_interval = opt.get("callback_interval", 50)
@@ -264,9 +267,9 @@ class WaveSimBase:
                     shape = [1, 1, 1]
                     shape[d] = -1
                     V = V * f.reshape(shape)
-        self.gamma = (1.0j / eps * V).astype(np.complex64)
-        self.k02e = np.complex64(k0c ** 2 / eps + 1.0j)
-        self.epsilon = np.float32(eps)
+        self.gamma = (1.0j / eps * V).astype(self.cdtype)                 # WaveSimBase.m:73
+        self.k02e = self.cdtype(k0c ** 2 / eps + 1.0j)                    # WaveSimBase.m:77
+        self.epsilon = self.rdtype(eps)                                   # WaveSimBase.m:78
         acc = np.ravel(opt.get("ACC", True))
         if acc.size == 1:
             acc = [not p for p in grid.periodic] if bool(acc[0]) else [False] * 3
@@ -281,14 +284,15 @@ class WaveSimBase:
 
     def _wiggles(self, table):
         g = self.grid
-        sq = np.sqrt(self.epsilon)  # the single-precision epsilon (WaveSimBase.m:78, 307-309)
+        sq = np.sqrt(self.epsilon)  # epsilon in the field's precision (WaveSimBase.m:78, 307-309)
+        rdt, cdt = self.rdtype, self.cdtype
         out = []
         for w in range(table.shape[1]):
             wy, wx, wz = table[:, w]
             def coord(pr, dp, s):
-                return ((pr - dp * s / 4).astype(np.float32) / sq).astype(np.float32)
+                return ((pr - dp * s / 4).astype(rdt) / sq).astype(rdt)
             def ramp(rng, s):
-                return np.exp(1.0j * math.pi / 2 * s * (rng / g.dx / len(rng))).astype(np.complex64)
+                return np.exp(1.0j * math.pi / 2 * s * (rng / g.dx / len(rng))).astype(cdt)
             out.append(dict(pxe=coord(g.px_range, g.dpx, wx), pye=coord(g.py_range, g.dpy, wy),
                             pze=coord(g.pz_range, g.dpz, wz), gx=ramp(g.x_range, wx),
                             gy=ramp(g.y_range, wy), gz=ramp(g.z_range, wz)))
@@ -301,7 +305,7 @@ class WaveSimBase:
                 self._solver.close()
             self._solver = MBSSolver(self.N, self.gamma, self.wiggles, float(self.epsilon),
                                      complex(self.k02e), self.roi, self.energy_calculation_interval,
-                                     nrhs=nrhs)
+                                     nrhs=nrhs, dtype=self.cdtype)
             self._solver_nrhs = nrhs
             plan = plans.lookup(self.N, nrhs, len(self.wiggles))
             if plan:
@@ -312,6 +316,8 @@ class WaveSimBase:
         """Take over another simulation's solver (same grid, wiggle count and
         ROI), loading this medium into it instead of allocating a new one."""
         sv = other._solver
+        if not self.single or (sv is not None and sv.f64):
+            raise NotImplementedError("complex128 solvers take no new medium: create a solver per medium")
         if sv is None or list(sv.N) != list(self.N) or sv.nw != len(self.wiggles) \
                 or not np.array_equal(other.roi, self.roi):
             raise ValueError("solver shape does not match this simulation")
@@ -332,7 +338,7 @@ class WaveSimBase:
     def _prepare(self, source: Source) -> Source:
         """Simulation.m:154-156: shift to the ROI origin and crop to the grid."""
         src = source.shift(self.roi[0]).crop([1, 1, 1, 1], self.N)
-        src.values = [v.astype(np.complex64) for v in src.values]
+        src.values = [v.astype(self.cdtype) for v in src.values]
         return src
 
     def max_iterations(self) -> int:

@@ -25,29 +25,40 @@ class RhsResult:
 
 
 class MBSSolver:
+    """dtype: np.complex64 (the reference's default, single_precision=true)
+    or np.complex128 (single_precision=false, Simulation.m:31, 221-252): the
+    fields, gamma, the wiggle tables and the sources are held and iterated in
+    that precision on the GPU."""
+
     def __init__(self, N: Sequence[int], gamma, wiggles, epsilon: float, k02e: complex,
-                 roi, energy_interval: int = 8, nrhs: int = 1):
+                 roi, energy_interval: int = 8, nrhs: int = 1, dtype=np.complex64):
         L = capi.lib()
         N = [int(x) for x in N]
         if len(N) == 3:
             N = N + [1]
         self.N = N
         self.nrhs = int(nrhs)
-        g = capi.c64(np.asarray(gamma).reshape(N[:3]))
-        self._w = capi.Wiggles(wiggles)
-        d = capi.desc_t()
+        self.dtype = np.dtype(dtype).type
+        if self.dtype not in (np.complex64, np.complex128):
+            raise ValueError("dtype must be complex64 or complex128")
+        self.f64 = self.dtype is np.complex128
+        g = capi.c64(np.asarray(gamma).reshape(N[:3]), self.dtype)
+        self._w = capi.Wiggles(wiggles, self.dtype)
+        d = capi.desc_f64_t() if self.f64 else capi.desc_t()
         d.N = (C.c_int64 * 4)(*N)
         d.nrhs = self.nrhs
         d.nwiggles = self._w.n
         d.epsilon = float(epsilon)
-        d.k02e = capi.cfloat(float(np.real(k02e)), float(np.imag(k02e)))
+        ctype = capi.cdouble if self.f64 else capi.cfloat
+        d.k02e = ctype(float(np.real(k02e)), float(np.imag(k02e)))
         roi = np.asarray(roi, dtype=np.int64)
         for r in range(2):
             for c in range(4):
                 d.roi[r][c] = int(roi[r, c])
         d.energy_interval = int(energy_interval)
         h = C.c_void_p()
-        capi.check(L.wsim_solver_create(C.byref(h), C.byref(d), capi.ptr(g), self._w.arr))
+        create = L.wsim_solver_create_f64 if self.f64 else L.wsim_solver_create
+        capi.check(create(C.byref(h), C.byref(d), capi.ptr(g), self._w.arr))
         self._h = h
         self._lib = L
         self.nw = self._w.n
@@ -74,8 +85,9 @@ class MBSSolver:
     def set_source(self, rhs: int, positions, values):
         """positions: 1-based [y,x,z,pol] grid positions; values: arrays in
         [y,x,z,pol] index order (already shifted and cropped to the grid)."""
-        p = capi.Patches(positions, values)
-        capi.check(self._lib.wsim_solver_set_source(self._h, int(rhs), p.n, p.arr))
+        p = capi.Patches(positions, values, self.dtype)
+        fn = self._lib.wsim_solver_set_source_f64 if self.f64 else self._lib.wsim_solver_set_source
+        capi.check(fn(self._h, int(rhs), p.n, p.arr))
 
     # -- run ------------------------------------------------------------------
     def solve(self, max_iter: int, energy_threshold: float, batch: int = 0,
@@ -108,9 +120,10 @@ class MBSSolver:
         return RhsResult(int(n.value), cv.value == 1, de[:m], ea[:m], cv.value < 0)
 
     def field(self, rhs: int = 0) -> np.ndarray:
-        """E' (primed, full grid) as a [y,x,z,pol] complex64 array."""
-        out = np.empty(self.N, dtype=np.complex64, order="F")
-        capi.check(self._lib.wsim_solver_get_field(self._h, int(rhs), capi.ptr(out)))
+        """E' (primed, full grid) as a [y,x,z,pol] array of the solver's dtype."""
+        out = np.empty(self.N, dtype=self.dtype, order="F")
+        fn = self._lib.wsim_solver_get_field_f64 if self.f64 else self._lib.wsim_solver_get_field
+        capi.check(fn(self._h, int(rhs), capi.ptr(out)))
         return out
 
     def device_ptr(self, which: int, rhs: int = 0) -> int:
@@ -121,9 +134,10 @@ class MBSSolver:
     def propagate(self, wiggle: int, E) -> np.ndarray:
         """One application of the propagation operator (WaveSim.m:15-36 /
         WaveSimVector.m:19-51) of wiggle `wiggle` (0-based)."""
-        x = capi.c64(np.asarray(E).reshape(self.N))
-        out = np.empty(self.N, dtype=np.complex64, order="F")
-        capi.check(self._lib.wsim_solver_propagate(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
+        x = capi.c64(np.asarray(E).reshape(self.N), self.dtype)
+        out = np.empty(self.N, dtype=self.dtype, order="F")
+        fn = self._lib.wsim_solver_propagate_f64 if self.f64 else self._lib.wsim_solver_propagate
+        capi.check(fn(self._h, int(wiggle), capi.ptr(x), capi.ptr(out)))
         return out
 
     def set_medium(self, gamma, wiggles, epsilon: float, k02e: complex):
