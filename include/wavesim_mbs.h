@@ -203,6 +203,12 @@ int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, d
 int wsim_solver_tune(wsim_solver* s, int reps, char* plan_out, int plan_cap, double* ms_per_iteration);
 int wsim_solver_get_plan(wsim_solver* s, char* plan_out, int plan_cap);
 int wsim_solver_set_plan(wsim_solver* s, const char* plan);
+/* Number of the pass launches of one iteration that run kernels compiled with
+ * the FFT plan as compile-time constants (built for the shipped tuned plans;
+ * other shapes run the run-time-plan kernels, with the same results; the
+ * environment variable WSIM_FIXED=0 turns them off).  No reference
+ * counterpart (a property of this build's kernels). */
+int wsim_solver_fixed_launches(wsim_solver* s);
 /* A new medium on the same grid and wiggle count: gamma, epsilon, k02e and
  * the wiggle tables are replaced (as a new WaveSim would compute them,
  * WaveSimBase.m:40-82); every rhs loses its source, set them again.  Lets one

@@ -1,0 +1,56 @@
+"""Plan-specialised pass kernels (FFT plan as compile-time constants).
+
+The shipped tuned plans get kernels whose stage radices, spans, tile width and
+workgroup size are constants; every other shape runs the run-time-plan kernel.
+Both compute the same butterflies in the same order, so a solve gives the same
+iteration count and fields equal to rounding (the compiler may contract
+multiply-adds differently in the two builds).  Bar: relative L2 <= 1e-6
+between the two.  Against the oracle, the full-size tests of these workloads
+(test_gpu_fullsize.py: the 320^3 fixture, 1152^2, 320^2 and the first 48
+iterations at 192^3) run on the plan-specialised kernels, as the launch
+counts asserted here show.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# workload -> pass launches of one iteration with plan-specialised kernels
+# (3-D: fused first axis, x forward, z middle, x inverse; 2-D: fused, middle)
+SHIPPED = {"disordered_3d_256": 4, "spheres_3d_128": 4, "disordered_2d_1024": 2, "homogeneous_2d_256": 2}
+
+
+def _solve(name, fixed: bool, monkeypatch):
+    import torch
+    from wavesim_amd import WaveSim
+    from wavesim_amd.workloads import WORKLOADS
+    monkeypatch.setenv("WSIM_FIXED", "1" if fixed else "0")
+    nmap, opt, source = WORKLOADS[name]()
+    sim = WaveSim(nmap, opt)
+    sv = sim.solver(1)
+    src = sim._prepare(source)
+    sv.set_source(0, src.positions, src.values)
+    nfixed = sv.fixed_launches
+    sv.solve(sim.max_iterations(), sim.energy_threshold)
+    torch.cuda.synchronize()
+    res = sv.result(0)
+    E = sv.field(0)
+    sv.close()
+    return nfixed, res, E
+
+
+@pytest.mark.parametrize("name", sorted(SHIPPED))
+def test_fixed_kernels_match_run_time_plan_kernels(gpu, name, monkeypatch):
+    nf, rf, Ef = _solve(name, True, monkeypatch)
+    ng, rg, Eg = _solve(name, False, monkeypatch)
+    print(f"{name}: fixed launches {nf}, iterations {rf.iterations} / {rg.iterations}, "
+          f"rel-L2 {rel_l2(Ef, Eg):.2e}", flush=True)
+    assert nf == SHIPPED[name]
+    assert ng == 0
+    assert rf.iterations == rg.iterations
+    assert rel_l2(Ef, Eg) <= 1e-6
+    assert np.allclose(rf.diff_energy, rg.diff_energy, rtol=1e-5)

@@ -7,7 +7,9 @@ instructions are disabled on the device side
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``).  The FFT plans tuned
 on MI355X (``tuned_plans.json``) are compiled into the library as a table
 (``_build/shipped_plans.inc``), so every entry point -- the MEX-shaped
-``wsim_run_algorithm`` included -- starts from them.
+``wsim_run_algorithm`` included -- starts from them, and the pass launches of
+those plans get kernels compiled with the plan as compile-time constants
+(``fixed_plans.py``: one generated unit per launch shape plus a lookup table).
 """
 from __future__ import annotations
 
@@ -74,13 +76,21 @@ def shipped_plans_inc(path: str = os.path.join(HERE, "tuned_plans.json")) -> str
     return "\n".join(rows) + "\n"
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
-    os.makedirs(OBJ, exist_ok=True)
-    inc = os.path.join(OBJ, "shipped_plans.inc")
-    text = shipped_plans_inc()
-    if not os.path.exists(inc) or open(inc).read() != text:
-        with open(inc, "w") as f:
+def _write_if_changed(path: str, text: str) -> None:
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
             f.write(text)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    sys.path.insert(0, HERE)
+    try:
+        import fixed_plans
+    finally:
+        sys.path.pop(0)
+    os.makedirs(OBJ, exist_ok=True)
+    _write_if_changed(os.path.join(OBJ, "shipped_plans.inc"), shipped_plans_inc())
+    shapes = fixed_plans.shipped_shapes()
     units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), ["-I", OBJ]),
              (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
@@ -90,6 +100,13 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
                 units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
                               [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",
                                f"-DWSIM_EPI={e}", f"-DWSIM_C={c}", f"-DWSIM_COMPACT={k}"]))
+    for i, shape in enumerate(shapes):
+        src = os.path.join(OBJ, f"fixed_{i}.hip")
+        _write_if_changed(src, fixed_plans.kernel_unit(i, shape))
+        units.append((src, os.path.join(OBJ, f"fixed_{i}.o"), []))
+    table = os.path.join(OBJ, "fixed_table.hip")
+    _write_if_changed(table, fixed_plans.table_unit(shapes))
+    units.append((table, os.path.join(OBJ, "fixed_table.o"), []))
     fno_obj = os.path.join(OBJ, "fno.o")
     jobs = jobs or min(16, os.cpu_count() or 4)
     cmds = [[hipcc()] + _flags() + defs + ["-c", src, "-o", obj] for src, obj, defs in units]

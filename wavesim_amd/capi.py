@@ -98,6 +98,7 @@ SYMBOLS = {
     "wsim_solver_get_field_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "wsim_solver_propagate_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "wsim_solver_element_size": (C.c_int, [C.c_void_p]),
+    "wsim_solver_fixed_launches": (C.c_int, [C.c_void_p]),
     "wsim_solver_solve": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, C.c_char_p,
                                     C.c_double, CALLBACK, C.c_void_p]),
     "wsim_solver_result": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int),

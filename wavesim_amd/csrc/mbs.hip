@@ -428,6 +428,7 @@ static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, int family, in
 
 struct Launch {
   KernelFn fn;
+  KernelFn fn_generic;  // the run-time-plan kernel of the same pass (fn, unless fn is plan-specialised)
   dim3 grid;
   int threads;
   size_t lds;
@@ -694,6 +695,8 @@ struct Solver {
     const bool compact = !f64 && !(env_k && std::atoi(env_k) == 0) &&
                          tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, 1, threads_of(axis, contig));
     L.fn = select_kernel(contig, mode, pro, epi, C, f64 ? kFamilyF64 : (compact ? 1 : 0));
+    L.fn_generic = L.fn;
+    if (!f64) L.fn = fixed_kernel(L.fn, contig, mode, pro, epi, axis, L.args.ncol, compact ? 1 : 0);
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
@@ -715,6 +718,33 @@ struct Solver {
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)L.lds));
     return L;
+  }
+
+  // the plan-specialised kernel of a launch of exactly this shape, if one
+  // was compiled (shipped tuned plans), else `generic`; WSIM_FIXED=0 keeps
+  // the run-time-plan kernels
+  KernelFn fixed_kernel(KernelFn generic, bool contig, int mode, int pro, int epi, int axis, int ncol,
+                        int family) const {
+    const char* env = std::getenv("WSIM_FIXED");
+    if (env && std::atoi(env) == 0) return generic;
+    const AxisPlan& pl = tab[axis].plan;
+    const int thr = threads_of(axis, contig);
+    for (const FixedKernel* fk = fixed_kernel_table(); fk->get; ++fk) {
+      if (fk->contig != (contig ? 1 : 0) || fk->mode != mode || fk->pro != pro || fk->epi != epi || fk->C != C ||
+          fk->family != family || fk->n != N[axis] || fk->ncol != ncol || fk->threads != thr || fk->nst != pl.nst)
+        continue;
+      bool same = true;
+      for (int q = 0; q < pl.nst; ++q) same = same && fk->rad[q] == pl.f[q].R;
+      if (same) return fk->get();
+    }
+    return generic;
+  }
+
+  // launches of one iteration that run plan-specialised kernels
+  int fixed_launches() const {
+    int k = 0;
+    for (const Launch& L : iter) k += L.fn_generic != nullptr && L.fn != L.fn_generic;
+    return k;
   }
 
   // the last inverse pass also runs the next iteration's first forward pass
@@ -1801,7 +1831,7 @@ struct Group {
       const auto &A = ps[a]->iter, &B = ps[b]->iter;
       if (A.size() != B.size()) return false;
       for (size_t k = 0; k < A.size(); ++k)
-        if (A[k].fn != B[k].fn || A[k].threads != B[k].threads) return false;
+        if (A[k].fn_generic != B[k].fn_generic || A[k].threads != B[k].threads) return false;
       return true;
     };
     while (!todo.empty()) {
@@ -1836,6 +1866,9 @@ struct Group {
           a.grp_end = sb.d_end + k * M;
           tab[k * M + m] = a;
         }
+        // members of different shapes share the run-time-plan kernel
+        for (size_t m = 1; m < M; ++m)
+          if (ps[sb.members[m]]->iter[k].fn != G.fn) G.fn = G.fn_generic;
         G.grid = dim3((unsigned)x, y, 1);
         G.lds = lds;
         G.dargs = sb.d_tab + k * M;
@@ -2059,6 +2092,13 @@ int wsim_solver_propagate_f64(wsim_solver* s, int w, const wsim_cdouble* in, wsi
     require(s->s.f64, "this solver computes in complex64: use wsim_solver_propagate");
     s->s.propagate(w, in, out);
     return WSIM_OK;
+  });
+}
+
+int wsim_solver_fixed_launches(wsim_solver* s) {
+  return guard([&]() -> int {
+    require(s != nullptr, "null solver");
+    return s->s.fixed_launches();
   });
 }
 

@@ -65,3 +65,16 @@ WSIM_FOR_EACH_VARIANT(WSIM_DECLARE_KPTR)
 WSIM_FOR_EACH_VARIANT_F64(WSIM_DECLARE_KPTR_F64)
 
 }  // namespace wsim
+
+namespace wsim {
+// A pass kernel compiled with its FFT plan as compile-time constants
+// (FixedPlan): the host launches it in place of the run-time-plan kernel for
+// launches of exactly this shape.  The table (generated at build time from
+// the shipped tuned plans) ends with an entry whose `get` is null.
+struct FixedKernel {
+  int contig, mode, pro, epi, C, family, n, ncol, threads, nst;
+  int rad[MAXST];
+  KernelFn (*get)();
+};
+const FixedKernel* fixed_kernel_table();
+}  // namespace wsim

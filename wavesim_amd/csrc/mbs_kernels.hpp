@@ -38,6 +38,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fft_radix.hpp"
 
 namespace wsim {
@@ -66,6 +68,65 @@ struct AxisPlan {
   Stage f[MAXST];  // forward stages (sign -1)
   Stage i[MAXST];  // inverse stages (sign +1), radices in reverse order
 };
+
+// A stage whose numbers are compile-time constants (kernels specialised for
+// one plan): the same fields as Stage, so that divisions by Ns and nb become
+// multiply-shifts by constants and LDS offsets become instruction immediates.
+template <int R_, int NS, int NB, int TW, int PITCH>
+struct CStage {
+  static constexpr int R = R_, Ns = NS, nb = NB, tw = TW;
+  static constexpr int nbp = NB * PITCH, nsp = NS * PITCH;
+};
+__device__ __forceinline__ uint32_t div_nb(const Stage& s, uint32_t x) { return fdiv(x, s.fnb); }
+__device__ __forceinline__ uint32_t div_Ns(const Stage& s, uint32_t x) { return fdiv(x, s.fNs); }
+template <int R, int NS, int NB, int TW, int P>
+__device__ __forceinline__ uint32_t div_nb(const CStage<R, NS, NB, TW, P>&, uint32_t x) { return x / (uint32_t)NB; }
+template <int R, int NS, int NB, int TW, int P>
+__device__ __forceinline__ uint32_t div_Ns(const CStage<R, NS, NB, TW, P>&, uint32_t x) { return x / (uint32_t)NS; }
+
+// The stage sequence of a pass: read from PassArgs at run time (DynPlan, one
+// kernel for every plan), or fixed at compile time (FixedPlan: transform
+// length N, NCOL lines per workgroup, NTH threads, radices Rs in forward
+// order; the host launches it only for launches of exactly that shape).
+// kWaves: the waves per SIMD the kernel's registers must allow (the
+// workgroups its LDS lets a CU hold), a launch bound for the compiler.
+// kEarly: issue the first stage's HBM loads before the LDS tables are built
+// (pays where few workgroups share a CU and cannot hide each other's set-up).
+struct DynPlan {
+  static constexpr bool kFixed = false, kEarly = false;
+  static constexpr int kN = 0, kNcol = 0, kNT = 0, kWaves = 1;
+};
+template <int N, int NCOL, int NTH, int WAVES, bool EARLY, int... Rs>
+struct FixedPlan {
+  static constexpr bool kFixed = true, kEarly = EARLY;
+  static constexpr int kN = N, kNcol = NCOL, kNT = NTH, kWaves = WAVES;
+  static constexpr int S = (int)sizeof...(Rs);
+  static constexpr int r[S] = {Rs...};
+  // radix of stage i of the forward (INV false) or inverse (reversed) transform
+  static constexpr int rad(int i, bool inv) { return inv ? r[S - 1 - i] : r[i]; }
+  static constexpr int span(int i, bool inv) {
+    int p = 1;
+    for (int k = 0; k < i; ++k) p *= rad(k, inv);
+    return p;
+  }
+  // offset of stage i's twiddles in the axis table (the host's layout:
+  // (R-1)*Ns entries per stage, stages in order)
+  static constexpr int twoff(int i, bool inv) {
+    int t = 0;
+    for (int k = 0; k < i; ++k) t += (rad(k, inv) - 1) * span(k, inv);
+    return t;
+  }
+  template <int I, bool INV, int PITCH>
+  using St = CStage<rad(I, INV), span(I, INV), N / rad(I, INV), twoff(I, INV), PITCH>;
+};
+
+template <int B, int E, class Fn>
+__device__ __forceinline__ void static_for(Fn&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 // Iteration state shared by all kernels of the loop; only control_kernel
 // writes it (WaveSimBase.m:165-189, Simulation.m:193-217).
@@ -307,9 +368,12 @@ __device__ __forceinline__ double rcp_green(double x) { return 1.0 / x; }
 #ifndef WSIM_FUSED_CHUNK  // the same when the next forward pass is fused into the mix
 #define WSIM_FUSED_CHUNK 4
 #endif
+#ifndef WSIM_FIXED_FUSED_CHUNK  // the same in plan-specialised kernels (4 rows spill there)
+#define WSIM_FIXED_FUSED_CHUNK 2
+#endif
 
 // F: the scalar type of the field (float: complex64, double: complex128)
-template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float>
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float, class K = DynPlan>
 struct Pass {
   using V = vec_of<F>;
   using CI = ColInfoT<F>;
@@ -355,14 +419,40 @@ struct Pass {
   // its last dE' afterwards (finish_e_kernel).
   bool acc_e;
 
+  // tile geometry: compile-time constants of a fixed plan, else the launch's
+  // values (lines per workgroup, LDS pitch, threads)
+  static constexpr int kPitch =
+      K::kFixed ? (CONTIG ? K::kN + (K::kN >> WSIM_PAD_SHIFT) + 1 : C * K::kNcol + 1) : 0;
+  __device__ __forceinline__ int ncol() const {
+    if constexpr (K::kFixed) return K::kNcol; else return a.ncol;
+  }
+  __device__ __forceinline__ int nlen() const {
+    if constexpr (K::kFixed) return K::kN; else return a.n;
+  }
+  __device__ __forceinline__ int pitch() const {
+    if constexpr (K::kFixed) return kPitch; else return a.lds_pitch;
+  }
+  __device__ __forceinline__ uint32_t nthr() const {
+    if constexpr (K::kFixed) return (uint32_t)K::kNT; else return nt;
+  }
+  __device__ __forceinline__ uint32_t lgncol() const {
+    if constexpr (K::kFixed) return (uint32_t)(31 - __builtin_clz((unsigned)K::kNcol)); else return lg_ncol;
+  }
+  // a fixed plan knows at compile time that every thread has a butterfly in
+  // step `it` of a stage
+  template <class ST>
+  static constexpr bool full(int it) {
+    if constexpr (K::kFixed) return (it + 1) * K::kNT <= K::kNcol * ST::nb; else return false;
+  }
+
   // contig lines get one padding slot every 8 rows: the Stockham stores of
   // the early stages (Ns < 8) write rows R apart, which would otherwise land
   // on the same LDS banks
   // (24-bit products: LDS indices stay far below 2^24, and a 24-bit multiply
   // issues at full rate where a 32-bit one takes four cycles)
   __device__ __forceinline__ int lidx(int t, int c, int row) const {
-    return CONTIG ? (int)__umul24((uint32_t)(c * a.ncol + t), (uint32_t)a.lds_pitch) + row + (row >> WSIM_PAD_SHIFT)
-                  : (int)__umul24((uint32_t)row, (uint32_t)a.lds_pitch) + c * a.ncol + t;
+    return CONTIG ? (int)__umul24((uint32_t)(c * ncol() + t), (uint32_t)pitch()) + row + (row >> WSIM_PAD_SHIFT)
+                  : (int)__umul24((uint32_t)row, (uint32_t)pitch()) + c * ncol() + t;
   }
 
   // HBM element index of row `row` of a column whose row 0 is at `off`
@@ -385,14 +475,15 @@ struct Pass {
   }
 
   // butterfly b -> (column t, butterfly index j)
-  __device__ __forceinline__ void decode(uint32_t b, const Stage& s, int& t, int& j) const {
+  template <class ST>
+  __device__ __forceinline__ void decode(uint32_t b, const ST& s, int& t, int& j) const {
     if (CONTIG) {
-      const uint32_t q = fdiv(b, s.fnb);
+      const uint32_t q = div_nb(s, b);
       j = (int)(b - q * (uint32_t)s.nb);
       t = (int)q;
     } else {
-      t = (int)(b & (uint32_t)(a.ncol - 1));
-      j = (int)(b >> lg_ncol);
+      t = (int)(b & (uint32_t)(ncol() - 1));
+      j = (int)(b >> lgncol());
     }
   }
 
@@ -423,31 +514,36 @@ struct Pass {
     int dt, dj, dk;
     uint32_t dg;
   };
-  __device__ __forceinline__ Walk walk(const Stage& s) const {
+  template <class ST>
+  __device__ __forceinline__ Walk walk(const ST& s) const {
     Walk w;
     if (CONTIG) {
-      w.dt = (int)fdiv(nt, s.fnb);
-      w.dj = (int)nt - w.dt * s.nb;
+      w.dt = (int)div_nb(s, nthr());
+      w.dj = (int)nthr() - w.dt * s.nb;
     } else {
       w.dt = 0;
-      w.dj = (int)(nt >> lg_ncol);
+      w.dj = (int)(nthr() >> lgncol());
     }
-    w.dk = w.dj - (int)fdiv((uint32_t)w.dj, s.fNs) * s.Ns;
+    w.dk = w.dj - (int)div_Ns(s, (uint32_t)w.dj) * s.Ns;
     w.dg = (uint32_t)w.dj * (uint32_t)a.rstride;
     return w;
   }
-  __device__ __forceinline__ Cursor start(const Stage& s) const {
+  template <class ST>
+  __device__ __forceinline__ Cursor start(const ST& s) const {
     Cursor c;
     decode((uint32_t)tid, s, c.t, c.j);
-    c.k = c.j - (int)fdiv((uint32_t)c.j, s.fNs) * s.Ns;
+    c.k = c.j - (int)div_Ns(s, (uint32_t)c.j) * s.Ns;
     c.g = CONTIG ? 0u : cfix.off + (uint32_t)c.j * (uint32_t)a.rstride;
     return c;
   }
-  // b < ncol * nb
-  __device__ __forceinline__ bool live(const Cursor& c, const Stage& s) const {
-    return CONTIG ? c.t < a.ncol : c.j < s.nb;
+  // b < ncol * nb (b = tid + it*nt)
+  template <class ST>
+  __device__ __forceinline__ bool live(const Cursor& c, const ST& s, int it) const {
+    if (full<ST>(it)) return true;
+    return CONTIG ? c.t < ncol() : c.j < s.nb;
   }
-  __device__ __forceinline__ void step(Cursor& c, const Walk& w, const Stage& s) const {
+  template <class ST>
+  __device__ __forceinline__ void step(Cursor& c, const Walk& w, const ST& s) const {
     c.t += w.dt;
     c.j += w.dj;
     c.k += w.dk;
@@ -484,21 +580,26 @@ struct Pass {
     return cmul(v, mkv<V>(dr * s, -di * s));
   }
 
-  template <int R>
-  static constexpr int its() { return its_per_stage(EPT, C, R); }
+  // butterflies per thread in a stage of radix R: the register budget of the
+  // kernel family, or exactly what a fixed plan's tile needs
+  template <int R, class ST = Stage>
+  static constexpr int its() {
+    if constexpr (K::kFixed) return (K::kNcol * ST::nb + K::kNT - 1) / K::kNT;
+    else return its_per_stage(EPT, C, R);
+  }
 
   // ---- stage pieces (C components, radix R) --------------------------------
   // first stage from HBM.  Only loads here: a use of a loaded value inside
   // the guarded block would make every iteration wait for its own loads.
-  template <int R>
-  __device__ __forceinline__ void stage_load_global(V* v, const Stage& s) const {
-    constexpr int ITS = its<R>();
+  template <int R, class ST>
+  __device__ __forceinline__ void stage_load_global(V* v, const ST& s) const {
+    constexpr int ITS = its<R, ST>();
     const Walk w = walk(s);
     Cursor cu = start(s);
     const uint32_t gs = gstep(s.nb);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const bool lv = live(cu, s);
+      const bool lv = live(cu, s, it);
       const CI& ci = col(lv ? cu.t : 0);
       const bool ok = lv && ci.valid;
       const uint32_t g0 = gbase(cu, ci);
@@ -514,7 +615,7 @@ struct Pass {
       cu = start(s);
 #pragma unroll
       for (int it = 0; it < ITS; ++it) {
-        if (live(cu, s)) {
+        if (live(cu, s, it)) {
           const V rc = col(cu.t).rc;
 #pragma unroll
           for (int c = 0; c < C; ++c)
@@ -527,20 +628,45 @@ struct Pass {
     }
   }
 
+  // The first stage's loads issued before the column table is staged (fixed
+  // plans without a prologue), so that they are in flight while the tables
+  // are built: a line's offset and validity come from the tile geometry
+  // (line0: the workgroup's first line; col_ok: the strided thread's column
+  // lies inside the grid).  The same loads as stage_load_global.
+  template <int R, class ST>
+  __device__ __forceinline__ void stage_load_early(V* v, const ST& s, int line0, bool col_ok) const {
+    constexpr int ITS = its<R, ST>();
+    const Walk w = walk(s);
+    Cursor cu = start(s);
+    const uint32_t gs = gstep(s.nb);
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const bool lv = live(cu, s, it);
+      const bool ok = lv && (CONTIG ? line0 + cu.t < a.nlines : col_ok);
+      const uint32_t g0 = CONTIG ? (uint32_t)cu.t * (uint32_t)nlen() + (uint32_t)cu.j : cu.g;
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[(c * ITS + it) * R + r] = ok ? load_raw(g0 + (uint32_t)r * gs, c) : mkv<V>(F(0), F(0));
+      step(cu, w, s);
+    }
+  }
+
   // A wave with at least one live butterfly reads without branching per lane:
   // lanes past the tile (the idle lanes of a partly filled last step) read
   // butterfly 0's operands, whose results are never stored, so the reads of
   // a butterfly issue together.  Waves with no live butterfly (a stage with
   // fewer butterflies than threads) skip the reads.
-  template <int R, bool FIRST = false>
-  __device__ __forceinline__ void stage_load_lds(V* v, const Stage& s, LC2 tw) const {
-    constexpr int ITS = its<R>();
+  template <int R, bool FIRST = false, class ST>
+  __device__ __forceinline__ void stage_load_lds(V* v, const ST& s, LC2 tw) const {
+    constexpr int ITS = its<R, ST>();
     const Walk w = walk(s);
     Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      const bool lv = live(cu, s);
-      if (__builtin_amdgcn_ballot_w64(lv) != 0) {
+      const bool lv = live(cu, s, it);
+      if (full<ST>(it) || __builtin_amdgcn_ballot_w64(lv) != 0) {
         const int t = lv ? cu.t : 0, j = lv ? cu.j : 0, k = lv ? cu.k : 0;
         LC2 twk = tw + s.tw + k;  // table [r-1][k]: lanes with consecutive k read consecutive words
 #pragma unroll
@@ -560,21 +686,21 @@ struct Pass {
     }
   }
 
-  template <int R, int DIR>
+  template <int R, int DIR, class ST = Stage>
   __device__ __forceinline__ void stage_compute(V* v) const {
-    constexpr int ITS = its<R>();
+    constexpr int ITS = its<R, ST>();
 #pragma unroll
     for (int k = 0; k < C * ITS; ++k) dft<R, DIR>(v + k * R);
   }
 
-  template <int R>
-  __device__ __forceinline__ void stage_store_lds(const V* v, const Stage& s) const {
-    constexpr int ITS = its<R>();
+  template <int R, class ST>
+  __device__ __forceinline__ void stage_store_lds(const V* v, const ST& s) const {
+    constexpr int ITS = its<R, ST>();
     const Walk w = walk(s);
     Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it) {
-      if (live(cu, s)) {
+      if (live(cu, s, it)) {
         const int base = (cu.j - cu.k) * R + cu.k;
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -588,24 +714,28 @@ struct Pass {
   // last stage: Ns = n / R, outputs land on rows j + r*nb; E2 = the
   // epilogue applied on the way out (EPI, or EPI_NONE for the forward tail of
   // EPI_MIX_FWD)
-  template <int R, int E2 = EPI>
-  __device__ __forceinline__ void stage_store_global(const V* v, const Stage& s) {
-    constexpr int ITS = its<R>();
+  template <int R, int E2 = EPI, class ST>
+  __device__ __forceinline__ void stage_store_global(const V* v, const ST& s) {
+    constexpr int ITS = its<R, ST>();
     const Walk w = walk(s);
     Cursor cu = start(s);
     const uint32_t gs = gstep(s.nb);
 #pragma unroll
     for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
-      if (live(cu, s)) {
+      if (live(cu, s, it)) {
         const int t = cu.t, j = cu.j;
         const CI& ci = col(t);
         if (!ci.valid) continue;
         if (E2 == EPI_MIX || E2 == EPI_MIX_FWD) {
-          mix_rows<R>(v, it, t, j, s.nb, ci);
+          mix_rows<R, ST>(v, it, t, j, s.nb, ci);
+          // a fixed plan's steps are branch-free: keep the scheduler from
+          // overlapping one step's mix with the next step's operand loads,
+          // which would hold more registers than the occupancy allows
+          if constexpr (K::kFixed) __builtin_amdgcn_sched_barrier(0);
           continue;
         }
         if constexpr (E2 == EPI_PSTD) {
-          pstd_rows<R>(v, it, j, s.nb, ci);
+          pstd_rows<R, ST>(v, it, j, s.nb, ci);
           continue;
         }
         const uint32_t g0 = gbase(cu, ci);
@@ -631,13 +761,14 @@ struct Pass {
   // The operands of a chunk of rows are loaded before any of its stores:
   // a load after a store to a possibly aliasing address would wait for the
   // previous row's whole load-compute-store chain.
-  template <int R>
+  template <int R, class ST>
   __device__ __forceinline__ void mix_rows(const V* v, int it, int t, int j, int nb, const CI& ci) {
-    constexpr int ITS = its<R>();
+    constexpr int ITS = its<R, ST>();
     // rows per chunk: 4 for scalar fields (118 VGPRs in the fused variant,
     // four waves per SIMD; 2 rows ran the pass 4% slower), 1 per component
     // for vector fields
-    constexpr int CH0 = C == 1 ? (EPI == EPI_MIX_FWD ? WSIM_FUSED_CHUNK : WSIM_EPI_CHUNK) : 1;
+    constexpr int CH0 =
+        C == 1 ? (EPI == EPI_MIX_FWD ? (K::kFixed ? WSIM_FIXED_FUSED_CHUNK : WSIM_FUSED_CHUNK) : WSIM_EPI_CHUNK) : 1;
     constexpr int CH = CH0 < R ? CH0 : R;
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
@@ -696,10 +827,10 @@ struct Pass {
   // Etmp = lap E (+ A S on source lines); E_next = c2 E + c1 E_prev + c3 Etmp
   // into E_prev's buffer; energy partials of E_next - E exp(i dphi) in the ROI.
   // Operands of a chunk of rows are loaded before any store (as mix_rows).
-  template <int R>
+  template <int R, class ST>
   __device__ __forceinline__ void pstd_rows(const V* v, int it, int j, int nb, const CI& ci) {
     static_assert(!F64, "the PSTD step is built for complex64 fields only");
-    constexpr int ITS = its<R>();
+    constexpr int ITS = its<R, ST>();
     constexpr int CH = 4 < R ? 4 : R;
     const int rlo = a.roi_lo[a.axis], rhi = a.roi_hi[a.axis];
 #pragma unroll
@@ -744,14 +875,14 @@ struct Pass {
   }
 
   // Green's function on the forward last-stage outputs (rows j + r*nb)
-  template <int R>
-  __device__ __forceinline__ void stage_green(V* v, const Stage& s) const {
-    constexpr int ITS = its<R>();
+  template <int R, class ST>
+  __device__ __forceinline__ void stage_green(V* v, const ST& s) const {
+    constexpr int ITS = its<R, ST>();
     const Walk w = walk(s);
     Cursor cu = start(s);
 #pragma unroll
     for (int it = 0; it < ITS; ++it, step(cu, w, s)) {
-      if (live(cu, s)) {
+      if (live(cu, s, it)) {
         const int j = cu.j;
         const CI& ci = col(cu.t);
 #pragma unroll
@@ -812,9 +943,9 @@ struct Pass {
   }
 #endif
 
-template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float>
-__global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restrict__ ap) {
-  using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT, F>;
+template <bool CONTIG, int MODE, int PRO, int EPI, int C, int NT, int EPT, class F = float, class K = DynPlan>
+__global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs* __restrict__ ap) {
+  using P = Pass<CONTIG, MODE, PRO, EPI, C, NT, EPT, F, K>;
   using V = typename P::V;
   using CI = typename P::CI;
   using L2 = typename P::L2;
@@ -833,6 +964,10 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     ap += e;
   }
   const PassArgs& a = *ap;
+  if constexpr (K::kFixed) {
+    // the host launches a plan-specialised kernel only for its own shape
+    if (a.n != K::kN || a.ncol != K::kNcol || (int)blockDim.x != K::kNT || a.plan.nst != K::S) return;
+  }
   if (a.xcd) {
     // workgroups go to the 8 XCDs round-robin by index: give XCD x the
     // contiguous block range [x*q + min(x, r), +q + (x < r)) (a bijection)
@@ -865,9 +1000,6 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   L2 twi_l = (L2)(smem + off_twi);
   L2 lds = (L2)(smem + off_data);
   const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
-  for (int i = threadIdx.x; i < ntf; i += nthr) stl(twf_l, i, ldt(as<V>(a.twf) + i));
-  for (int i = threadIdx.x; i < nti; i += nthr) stl(twi_l, i, ldt(as<V>(a.twi) + i));
-
   const int it_cur = st->it;
   const int nw = st->nw;
   const int w = (it_cur - 1) % nw;
@@ -884,62 +1016,6 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
     vblk = (long long)o * a.n * a.inner + (long long)itile * a.ncol;
   }
 
-  for (int r = threadIdx.x; r < a.n; r += nthr) {
-    stl(ramp_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w * a.n + r));
-    if (FUSE) stl(ramp2_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w2 * a.n + r));
-    pe_l[r] = ldt(as<F>(a.pe[a.axis]) + (size_t)w * a.n + r);
-  }
-  // per-column coordinates (axes other than a.axis)
-  for (int t = threadIdx.x; t < a.ncol; t += nthr) {
-    CI ci;
-    int crd[3] = {0, 0, 0};
-    int valid;
-    if (CONTIG) {
-      const int line = bx * a.ncol + t;  // < nvox < 2^31
-      valid = line < a.nlines;
-      ci.off = (uint32_t)(t * a.n);
-      int rem = valid ? line : 0;
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
-    } else {
-      const int i = itile * a.ncol + t;
-      valid = i < a.inner;
-      ci.off = (uint32_t)t;
-      int rem = valid ? i : 0;
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        if (b < a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
-      rem = o;
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
-    }
-    ci.valid = valid;
-    ci.src = 0;
-    if (CONTIG && a.srcline != nullptr && valid) ci.src = a.srcline[bx * a.ncol + t];
-    V rc = mkv<V>(F(1), F(0)), rc2 = mkv<V>(F(1), F(0));
-    F q = F(0);
-    int roi_ok = 1;
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      ci.p[b] = F(0);
-      if (b == a.axis) continue;
-      const F pb = ldt(as<F>(a.pe[b]) + (size_t)w * a.N[b] + crd[b]);
-      ci.p[b] = pb;
-      q += pb * pb;
-      rc = cmul(rc, ldt(as<V>(a.ramp[b]) + (size_t)w * a.N[b] + crd[b]));
-      if (FUSE) rc2 = cmul(rc2, ldt(as<V>(a.ramp[b]) + (size_t)w2 * a.N[b] + crd[b]));
-      roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
-    }
-    ci.rc = rc;
-    ci.rc2 = rc2;
-    ci.q = q;
-    ci.roi_ok = roi_ok;
-    cols[t] = ci;
-  }
-  __syncthreads();
-
   P p{a, lds, cols, (int)threadIdx.x, a.S != nullptr && it_cur <= nw,
       st->warm != 0 && a.W0 != nullptr && it_cur <= nw, F(1) / F(nw), 0u, ramp_l, ramp2_l,
       FUSE && a.S != nullptr && it_cur + 1 <= nw, FUSE && st->warm != 0 && a.W0 != nullptr && it_cur + 1 <= nw,
@@ -947,10 +1023,7 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   p.eacc = 0.0;
   p.acc_e = (it_cur & 1) == 0;
   p.nt = (uint32_t)nthr;
-  if (!CONTIG) {
-    p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
-    p.cfix = cols[threadIdx.x & (a.ncol - 1)];
-  }
+  if (!CONTIG) p.lg_ncol = 31 - __builtin_clz((unsigned)a.ncol);
   const long long rbase = (long long)rhs * a.rhs_stride + vblk;
   // PSTD: the first pass transforms the current time level
   const V* in_sel = as<V>((PRO == PRO_SRC_RAMP && a.pstd) ? a.fld[(it_cur - 1) & 1] : a.in);
@@ -978,124 +1051,310 @@ __global__ void __launch_bounds__(NT) axis_pass_kernel(const PassArgs* __restric
   }
   p.gam_b = wave_uniform(as<V>(a.gamma) + vblk);
 
-  const AxisPlan& pl = a.plan;
-  const int S = pl.nst;
-  constexpr bool MID = MODE == MODE_MID;
-  const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
-  LC2 tw = (MODE == MODE_INV) ? twi_l : twf_l;
-  constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
-
-  // Each region below is its own radix switch, so the register allocator
-  // sees one short straight-line path per radix.
-  if (S == 1) {
-    // single stage: HBM -> registers -> HBM (the middle pass applies the
-    // Green's function and the inverse DFT in registers)
-    WSIM_RADIX_SWITCH(fs[0].R,
-      V v[P::template its<R>() * C * R];
-      p.template stage_load_global<R>(v, fs[0]);
-      p.template stage_compute<R, D0>(v);
-      if (MID) {
-        p.template stage_green<R>(v, fs[0]);
-        p.template stage_compute<R, 1>(v);
-        p.template stage_store_global<R>(v, pl.i[0]);
-      } else {
-        p.template stage_store_global<R>(v, fs[0]);
-      }
-    )
-  } else {
-    // first stage: HBM -> registers -> LDS
-    WSIM_RADIX_SWITCH(fs[0].R,
-      V v[P::template its<R>() * C * R];
-      p.template stage_load_global<R>(v, fs[0]);
-      p.template stage_compute<R, D0>(v);
-      p.template stage_store_lds<R>(v, fs[0]);
-    )
-    __syncthreads();
-    // middle stages: LDS -> registers -> LDS
-    for (int s = 1; s < S - 1; ++s) {
-      const Stage st_ = fs[s];
-      WSIM_RADIX_SWITCH(st_.R,
-        V v[P::template its<R>() * C * R];
-        p.template stage_load_lds<R>(v, st_, tw);
-        __syncthreads();
-        p.template stage_compute<R, D0>(v);
-        p.template stage_store_lds<R>(v, st_);
-      )
-      __syncthreads();
+  // the LDS tables: twiddles, this axis's ramps and coordinates, and the
+  // per-column coordinates (axes other than a.axis)
+  auto setup_tables = [&]() {
+    for (int i = threadIdx.x; i < ntf; i += nthr) stl(twf_l, i, ldt(as<V>(a.twf) + i));
+    for (int i = threadIdx.x; i < nti; i += nthr) stl(twi_l, i, ldt(as<V>(a.twi) + i));
+    for (int r = threadIdx.x; r < a.n; r += nthr) {
+      stl(ramp_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w * a.n + r));
+      if (FUSE) stl(ramp2_l, r, ldt(as<V>(a.ramp[a.axis]) + (size_t)w2 * a.n + r));
+      pe_l[r] = ldt(as<F>(a.pe[a.axis]) + (size_t)w * a.n + r);
     }
-    const Stage sl = fs[S - 1];
-    if (!MID) {
-      // last stage: LDS -> registers -> HBM
-      WSIM_RADIX_SWITCH(sl.R,
-        V v[P::template its<R>() * C * R];
-        p.template stage_load_lds<R>(v, sl, tw);
-        if (FUSE) __syncthreads();  // the epilogue writes the next pass's operand to LDS
-        p.template stage_compute<R, D0>(v);
-        p.template stage_store_global<R>(v, sl);
-      )
-      if (FUSE) {
-        // the next iteration's forward transform of the same lines: LDS ->
-        // forward stages -> work field (the plan has at least two stages)
-        __syncthreads();
-        const Stage* ff = pl.f;
-        WSIM_RADIX_SWITCH(ff[0].R,
-          V v[P::template its<R>() * C * R];
-          p.template stage_load_lds<R, true>(v, ff[0], twf_l);
-          __syncthreads();
-          p.template stage_compute<R, -1>(v);
-          p.template stage_store_lds<R>(v, ff[0]);
-        )
-        __syncthreads();
-        for (int s = 1; s < S - 1; ++s) {
-          const Stage st_ = ff[s];
-          WSIM_RADIX_SWITCH(st_.R,
-            V v[P::template its<R>() * C * R];
-            p.template stage_load_lds<R>(v, st_, twf_l);
-            __syncthreads();
-            p.template stage_compute<R, -1>(v);
-            p.template stage_store_lds<R>(v, st_);
-          )
-          __syncthreads();
-        }
-        const Stage fl = ff[S - 1];
-        WSIM_RADIX_SWITCH(fl.R,
-          V v[P::template its<R>() * C * R];
-          p.template stage_load_lds<R>(v, fl, twf_l);
-          p.template stage_compute<R, -1>(v);
-          p.template stage_store_global<R, EPI_NONE>(v, fl);
-        )
+    // per-column coordinates (axes other than a.axis)
+    for (int t = threadIdx.x; t < a.ncol; t += nthr) {
+      CI ci;
+      int crd[3] = {0, 0, 0};
+      int valid;
+      if (CONTIG) {
+        const int line = bx * a.ncol + t;  // < nvox < 2^31
+        valid = line < a.nlines;
+        ci.off = (uint32_t)(t * a.n);
+        int rem = valid ? line : 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
+      } else {
+        const int i = itile * a.ncol + t;
+        valid = i < a.inner;
+        ci.off = (uint32_t)t;
+        int rem = valid ? i : 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          if (b < a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
+        rem = o;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          if (b > a.axis) { crd[b] = rem % a.N[b]; rem /= a.N[b]; }
+      }
+      ci.valid = valid;
+      ci.src = 0;
+      if (CONTIG && a.srcline != nullptr && valid) ci.src = a.srcline[bx * a.ncol + t];
+      V rc = mkv<V>(F(1), F(0)), rc2 = mkv<V>(F(1), F(0));
+      F q = F(0);
+      int roi_ok = 1;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        ci.p[b] = F(0);
+        if (b == a.axis) continue;
+        const F pb = ldt(as<F>(a.pe[b]) + (size_t)w * a.N[b] + crd[b]);
+        ci.p[b] = pb;
+        q += pb * pb;
+        rc = cmul(rc, ldt(as<V>(a.ramp[b]) + (size_t)w * a.N[b] + crd[b]));
+        if (FUSE) rc2 = cmul(rc2, ldt(as<V>(a.ramp[b]) + (size_t)w2 * a.N[b] + crd[b]));
+        roi_ok &= (crd[b] >= a.roi_lo[b]) & (crd[b] <= a.roi_hi[b]);
+      }
+      ci.rc = rc;
+      ci.rc2 = rc2;
+      ci.q = q;
+      ci.roi_ok = roi_ok;
+      cols[t] = ci;
+    }
+  };
+
+  constexpr bool MID = MODE == MODE_MID;
+  constexpr int D0 = (MODE == MODE_INV) ? 1 : -1;
+  if constexpr (K::kFixed) {
+    // the plan's stages as compile-time constants: the same stage pieces
+    // and order as the run-time sequence below
+    constexpr int S = K::S;
+    constexpr bool FI = MODE == MODE_INV;  // the first transform runs the inverse stages
+    constexpr int PT = P::kPitch;
+    LC2 tw = FI ? twi_l : twf_l;
+    using S0 = typename K::template St<0, FI, PT>;
+    V v0[P::template its<S0::R, S0>() * C * S0::R];
+    constexpr bool EARLY = PRO == PRO_NONE && K::kEarly;
+    if constexpr (EARLY) {
+      const int t0 = (int)(threadIdx.x & (uint32_t)(K::kNcol - 1));
+      if (!CONTIG) p.cfix.off = (uint32_t)t0;
+      p.template stage_load_early<S0::R>(v0, S0{}, CONTIG ? bx * K::kNcol : 0,
+                                         !CONTIG && itile * K::kNcol + t0 < a.inner);
+    }
+    setup_tables();
+    __syncthreads();
+    if (!CONTIG) p.cfix = cols[threadIdx.x & (uint32_t)(K::kNcol - 1)];
+    if constexpr (!EARLY) p.template stage_load_global<S0::R>(v0, S0{});
+    if constexpr (S == 1) {
+      constexpr int R = S0::R;
+      V* v = v0;
+      p.template stage_compute<R, D0, S0>(v);
+      if constexpr (MID) {
+        using I0 = typename K::template St<0, true, PT>;
+        p.template stage_green<R>(v, S0{});
+        p.template stage_compute<R, 1, S0>(v);
+        p.template stage_store_global<R>(v, I0{});
+      } else {
+        p.template stage_store_global<R>(v, S0{});
       }
     } else {
-      // forward last stage, Green's function, inverse first stage (same
-      // radix, Ns = 1, no twiddles), back to LDS
-      WSIM_RADIX_SWITCH(sl.R,
-        V v[P::template its<R>() * C * R];
-        p.template stage_load_lds<R>(v, sl, tw);
+      {
+        constexpr int R = S0::R;
+        V* v = v0;
+        p.template stage_compute<R, D0, S0>(v);
+        p.template stage_store_lds<R>(v, S0{});
+      }
+      __syncthreads();
+      static_for<1, S - 1>([&](auto I) {
+        using SI = typename K::template St<decltype(I)::value, FI, PT>;
+        constexpr int R = SI::R;
+        V v[P::template its<R, SI>() * C * R];
+        p.template stage_load_lds<R>(v, SI{}, tw);
         __syncthreads();
-        p.template stage_compute<R, -1>(v);
-        p.template stage_green<R>(v, sl);
-        p.template stage_compute<R, 1>(v);
-        p.template stage_store_lds<R>(v, pl.i[0]);
+        p.template stage_compute<R, D0, SI>(v);
+        p.template stage_store_lds<R>(v, SI{});
+        __syncthreads();
+      });
+      using SL = typename K::template St<S - 1, FI, PT>;
+      if constexpr (!MID) {
+        {
+          constexpr int R = SL::R;
+          V v[P::template its<R, SL>() * C * R];
+          p.template stage_load_lds<R>(v, SL{}, tw);
+          if (FUSE) __syncthreads();  // the epilogue writes the next pass's operand to LDS
+          p.template stage_compute<R, D0, SL>(v);
+          p.template stage_store_global<R>(v, SL{});
+        }
+        if constexpr (FUSE) {
+          __syncthreads();
+          {
+            using F0 = typename K::template St<0, false, PT>;
+            constexpr int R = F0::R;
+            V v[P::template its<R, F0>() * C * R];
+            p.template stage_load_lds<R, true>(v, F0{}, twf_l);
+            __syncthreads();
+            p.template stage_compute<R, -1, F0>(v);
+            p.template stage_store_lds<R>(v, F0{});
+          }
+          __syncthreads();
+          static_for<1, S - 1>([&](auto I) {
+            using FI_ = typename K::template St<decltype(I)::value, false, PT>;
+            constexpr int R = FI_::R;
+            V v[P::template its<R, FI_>() * C * R];
+            p.template stage_load_lds<R>(v, FI_{}, twf_l);
+            __syncthreads();
+            p.template stage_compute<R, -1, FI_>(v);
+            p.template stage_store_lds<R>(v, FI_{});
+            __syncthreads();
+          });
+          using FL = typename K::template St<S - 1, false, PT>;
+          constexpr int R = FL::R;
+          V v[P::template its<R, FL>() * C * R];
+          p.template stage_load_lds<R>(v, FL{}, twf_l);
+          p.template stage_compute<R, -1, FL>(v);
+          p.template stage_store_global<R, EPI_NONE>(v, FL{});
+        }
+      } else {
+        {
+          using I0 = typename K::template St<0, true, PT>;
+          constexpr int R = SL::R;
+          V v[P::template its<R, SL>() * C * R];
+          p.template stage_load_lds<R>(v, SL{}, tw);
+          __syncthreads();
+          p.template stage_compute<R, -1, SL>(v);
+          p.template stage_green<R>(v, SL{});
+          p.template stage_compute<R, 1, SL>(v);
+          p.template stage_store_lds<R>(v, I0{});
+        }
+        __syncthreads();
+        static_for<1, S - 1>([&](auto I) {
+          using II = typename K::template St<decltype(I)::value, true, PT>;
+          constexpr int R = II::R;
+          V v[P::template its<R, II>() * C * R];
+          p.template stage_load_lds<R>(v, II{}, twi_l);
+          __syncthreads();
+          p.template stage_compute<R, 1, II>(v);
+          p.template stage_store_lds<R>(v, II{});
+          __syncthreads();
+        });
+        using IL = typename K::template St<S - 1, true, PT>;
+        constexpr int R = IL::R;
+        V v[P::template its<R, IL>() * C * R];
+        p.template stage_load_lds<R>(v, IL{}, twi_l);
+        p.template stage_compute<R, 1, IL>(v);
+        p.template stage_store_global<R>(v, IL{});
+      }
+    }
+  } else {
+    setup_tables();
+    __syncthreads();
+    if (!CONTIG) p.cfix = cols[threadIdx.x & (a.ncol - 1)];
+    const AxisPlan& pl = a.plan;
+    const int S = pl.nst;
+    const Stage* fs = (MODE == MODE_INV) ? pl.i : pl.f;
+    LC2 tw = (MODE == MODE_INV) ? twi_l : twf_l;
+
+    // Each region below is its own radix switch, so the register allocator
+    // sees one short straight-line path per radix.
+    if (S == 1) {
+      // single stage: HBM -> registers -> HBM (the middle pass applies the
+      // Green's function and the inverse DFT in registers)
+      WSIM_RADIX_SWITCH(fs[0].R,
+        V v[P::template its<R>() * C * R];
+        p.template stage_load_global<R>(v, fs[0]);
+        p.template stage_compute<R, D0>(v);
+        if (MID) {
+          p.template stage_green<R>(v, fs[0]);
+          p.template stage_compute<R, 1>(v);
+          p.template stage_store_global<R>(v, pl.i[0]);
+        } else {
+          p.template stage_store_global<R>(v, fs[0]);
+        }
+      )
+    } else {
+      // first stage: HBM -> registers -> LDS
+      WSIM_RADIX_SWITCH(fs[0].R,
+        V v[P::template its<R>() * C * R];
+        p.template stage_load_global<R>(v, fs[0]);
+        p.template stage_compute<R, D0>(v);
+        p.template stage_store_lds<R>(v, fs[0]);
       )
       __syncthreads();
+      // middle stages: LDS -> registers -> LDS
       for (int s = 1; s < S - 1; ++s) {
-        const Stage st_ = pl.i[s];
+        const Stage st_ = fs[s];
         WSIM_RADIX_SWITCH(st_.R,
           V v[P::template its<R>() * C * R];
-          p.template stage_load_lds<R>(v, st_, twi_l);
+          p.template stage_load_lds<R>(v, st_, tw);
           __syncthreads();
-          p.template stage_compute<R, 1>(v);
+          p.template stage_compute<R, D0>(v);
           p.template stage_store_lds<R>(v, st_);
         )
         __syncthreads();
       }
-      const Stage il = pl.i[S - 1];
-      WSIM_RADIX_SWITCH(il.R,
-        V v[P::template its<R>() * C * R];
-        p.template stage_load_lds<R>(v, il, twi_l);
-        p.template stage_compute<R, 1>(v);
-        p.template stage_store_global<R>(v, il);
-      )
+      const Stage sl = fs[S - 1];
+      if (!MID) {
+        // last stage: LDS -> registers -> HBM
+        WSIM_RADIX_SWITCH(sl.R,
+          V v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R>(v, sl, tw);
+          if (FUSE) __syncthreads();  // the epilogue writes the next pass's operand to LDS
+          p.template stage_compute<R, D0>(v);
+          p.template stage_store_global<R>(v, sl);
+        )
+        if (FUSE) {
+          // the next iteration's forward transform of the same lines: LDS ->
+          // forward stages -> work field (the plan has at least two stages)
+          __syncthreads();
+          const Stage* ff = pl.f;
+          WSIM_RADIX_SWITCH(ff[0].R,
+            V v[P::template its<R>() * C * R];
+            p.template stage_load_lds<R, true>(v, ff[0], twf_l);
+            __syncthreads();
+            p.template stage_compute<R, -1>(v);
+            p.template stage_store_lds<R>(v, ff[0]);
+          )
+          __syncthreads();
+          for (int s = 1; s < S - 1; ++s) {
+            const Stage st_ = ff[s];
+            WSIM_RADIX_SWITCH(st_.R,
+              V v[P::template its<R>() * C * R];
+              p.template stage_load_lds<R>(v, st_, twf_l);
+              __syncthreads();
+              p.template stage_compute<R, -1>(v);
+              p.template stage_store_lds<R>(v, st_);
+            )
+            __syncthreads();
+          }
+          const Stage fl = ff[S - 1];
+          WSIM_RADIX_SWITCH(fl.R,
+            V v[P::template its<R>() * C * R];
+            p.template stage_load_lds<R>(v, fl, twf_l);
+            p.template stage_compute<R, -1>(v);
+            p.template stage_store_global<R, EPI_NONE>(v, fl);
+          )
+        }
+      } else {
+        // forward last stage, Green's function, inverse first stage (same
+        // radix, Ns = 1, no twiddles), back to LDS
+        WSIM_RADIX_SWITCH(sl.R,
+          V v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R>(v, sl, tw);
+          __syncthreads();
+          p.template stage_compute<R, -1>(v);
+          p.template stage_green<R>(v, sl);
+          p.template stage_compute<R, 1>(v);
+          p.template stage_store_lds<R>(v, pl.i[0]);
+        )
+        __syncthreads();
+        for (int s = 1; s < S - 1; ++s) {
+          const Stage st_ = pl.i[s];
+          WSIM_RADIX_SWITCH(st_.R,
+            V v[P::template its<R>() * C * R];
+            p.template stage_load_lds<R>(v, st_, twi_l);
+            __syncthreads();
+            p.template stage_compute<R, 1>(v);
+            p.template stage_store_lds<R>(v, st_);
+          )
+          __syncthreads();
+        }
+        const Stage il = pl.i[S - 1];
+        WSIM_RADIX_SWITCH(il.R,
+          V v[P::template its<R>() * C * R];
+          p.template stage_load_lds<R>(v, il, twi_l);
+          p.template stage_compute<R, 1>(v);
+          p.template stage_store_global<R>(v, il);
+        )
+      }
     }
   }
 

@@ -184,6 +184,14 @@ class MBSSolver:
     def plan(self, text: str):
         capi.check(self._lib.wsim_solver_set_plan(self._h, text.encode()))
 
+    @property
+    def fixed_launches(self) -> int:
+        """Pass launches of one iteration that run plan-specialised kernels."""
+        n = self._lib.wsim_solver_fixed_launches(self._h)
+        if n < 0:
+            capi.check(n)
+        return n
+
     def profile(self, reps: int = 10):
         """[(kernel name, mean ms, algorithmic bytes)] for one iteration."""
         K = 16
