@@ -192,10 +192,14 @@ int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, d
                         const char** names, int* nkernels);
 /* FFT plan of every non-trivial axis, as text "y:5,4,16/16/512;x:5,16,4/16/0/xcd;..."
  * (radices of the Stockham stages in forward order / workgroup columns,
- * 0 = widest that fits / optional workgroup size, 256 or 512, 0 = default /
+ * 0 = widest that fits / optional workgroup size, a multiple of 64 up to
+ * 512 (strided axes: also a multiple of the workgroup columns), 0 = default /
  * optional "xcd": each of the 8 XCDs takes one contiguous range of the
  * axis's tiles instead of every 8th tile).  Radices are 2, 3, 4, 5, 7, 8,
- * 11, 16 (16 not for 3-component fields).  A plan changes speed only.
+ * 11, 16 (16 not for 3-component fields), and 20 in launch shapes that have
+ * a plan-specialised kernel (wsim_solver_fixed_launches).  A plan changes
+ * speed only.  A plan whose launches cannot be built returns an error and
+ * leaves the previous plan in force.
  * wsim_solver_tune times whole iterations of this problem for candidate radix
  * orders, tile widths, workgroup sizes and tile ranges, axis by axis, keeps
  * the fastest and reports it (the reference picks FFT-friendly sizes only,

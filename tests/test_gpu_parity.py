@@ -464,9 +464,30 @@ def test_plan_rejects_bad_text(gpu):
     n, opt, _ = cases.case_3d_disordered()
     _, g = _pair(n, opt)
     sv = g.solver()
-    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y", "y:4,7/0/256/xdc", "y:4,7/0/128", "y:4,7/0/0/"):
+    for bad in ("q:4,7/0", "y:4,8/0", "y:6,14/0", "y", "y:4,7/0/256/xdc", "y:4,7/0/100", "y:4,7/0/576",
+                "y:4,7/0/0/"):
         with pytest.raises(WsimError):
             sv.plan = bad
+
+
+def test_plan_with_radix_20_needs_a_specialised_kernel(gpu):
+    """Radix 20 runs in plan-specialised kernels only: on a shape without one
+    the plan is refused with WSIM_ERR_UNSUPPORTED, the previous plan stays in
+    force, and the solve still matches the oracle."""
+    from oracle import mbs
+    from wavesim_amd.capi import WsimError, WSIM_ERR_UNSUPPORTED
+    n, opt, src = cases.case_2d_homogeneous(n=64, b=8)
+    o, g = _pair(n, opt)
+    sv = g.solver()
+    assert sv.N[0] == 80
+    before = sv.plan
+    with pytest.raises(WsimError) as e:
+        sv.plan = "y:20,4/0"
+    assert e.value.code == WSIM_ERR_UNSUPPORTED
+    assert sv.plan == before
+    Eo, so = mbs.run(o, cases.oracle_source(src), record_every=True)
+    Eg, sg = g.exec(cases.gpu_source(src))
+    _check_solve(Eo, so, Eg, sg)
 
 
 @pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_periodic_2d, cases.case_3d_disordered])

@@ -19,6 +19,9 @@ def test_pass_names_from_template_arguments():
     assert short_name("void wsim::axis_pass_kernel<false, 2, 0, 0, 3, 256, 24>(x)") == "pass_mid_green[C=3]"
     assert short_name("void wsim::axis_pass_kernel<true, 1, 0, 3, 1, 512, 16>(wsim::PassArgs const*)") \
         == "pass_inv_mix_fwd[C=1]"
+    assert short_name("void wsim::axis_pass_kernel<false, 2, 0, 0, 1, 320, 16, float, "
+                      "wsim::FixedPlan<320, 16, 320, 3, false, 20, 16> >(wsim::PassArgs const*)") \
+        == "pass_mid_green[C=1,fixed]"
     assert short_name("wsim::control_kernel(wsim::DevState*, double const*, int)") == "control"
 
 

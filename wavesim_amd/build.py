@@ -90,7 +90,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         sys.path.pop(0)
     os.makedirs(OBJ, exist_ok=True)
     _write_if_changed(os.path.join(OBJ, "shipped_plans.inc"), shipped_plans_inc())
-    shapes = fixed_plans.shipped_shapes()
+    shapes = fixed_plans.shipped_shapes(extra=os.environ.get("WSIM_FIXED_EXTRA", ""))
     units = [(os.path.join(CSRC, "mbs.hip"), os.path.join(OBJ, "mbs.o"), ["-I", OBJ]),
              (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
@@ -104,6 +104,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         src = os.path.join(OBJ, f"fixed_{i}.hip")
         _write_if_changed(src, fixed_plans.kernel_unit(i, shape))
         units.append((src, os.path.join(OBJ, f"fixed_{i}.o"), []))
+    for f in os.listdir(OBJ):  # units of shapes no longer generated
+        m = re.fullmatch(r"fixed_(\d+)\.(hip|o)", f)
+        if m and int(m.group(1)) >= len(shapes):
+            os.remove(os.path.join(OBJ, f))
     table = os.path.join(OBJ, "fixed_table.hip")
     _write_if_changed(table, fixed_plans.table_unit(shapes))
     units.append((table, os.path.join(OBJ, "fixed_table.o"), []))

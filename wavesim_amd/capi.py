@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WSIM_LIB", os.path.join(_HERE, "libwavesim_mbs.so"))
 
 WSIM_OK = 0
+WSIM_ERR_UNSUPPORTED = -3
 ERRORS = {-1: "WSIM_ERR_ARG", -2: "WSIM_ERR_HIP", -3: "WSIM_ERR_UNSUPPORTED", -4: "WSIM_ERR_NOMEM",
           -5: "WSIM_ERR_ABORTED", -6: "WSIM_ERR_DIVERGED"}
 WSIM_ERR_ABORTED = -5  # the callback asked to stop

@@ -196,12 +196,55 @@ __device__ __forceinline__ void dft_odd(V* v) {
   }
 }
 
+// radix 20 = 4 x 5 in registers (Cooley-Tukey, n = 5 n1 + n2, k = k1 + 4 k2):
+// 4-point DFTs over n1, twiddles w20^(n2 k1), 5-point DFTs over n2.  Two
+// stages of 16 and 20 make a 320-point transform with one LDS exchange.
+template <int DIR, class V>
+__device__ __forceinline__ void dft20(V* v) {
+  using S = real_of<V>;
+  V a[5][4];  // a[n2][k1]
+#pragma unroll
+  for (int n2 = 0; n2 < 5; ++n2) {
+    a[n2][0] = v[n2];
+    a[n2][1] = v[n2 + 5];
+    a[n2][2] = v[n2 + 10];
+    a[n2][3] = v[n2 + 15];
+    dft4<DIR>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+  }
+  // cos/sin(2 pi m / 20), m = 0..5 (m = n2 k1 <= 12 folds onto them)
+  constexpr double c20[6] = {1.0, 0.95105651629515357, 0.80901699437494742, 0.58778525229247314,
+                             0.30901699437494745, 0.0};
+#pragma unroll
+  for (int n2 = 1; n2 < 5; ++n2) {
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) {
+      const int m = n2 * k1;  // 1..12
+      // exp(DIR i 2 pi m / 20) from the first quadrant
+      const int q = m / 5, r = m % 5;
+      const S cr = S(c20[r]), sr = S(c20[5 - r]);
+      S c, sn;
+      if (q == 0) { c = cr; sn = sr; }
+      else if (q == 1) { c = -sr; sn = cr; }
+      else { c = -cr; sn = -sr; }
+      a[n2][k1] = cmul(a[n2][k1], mkv<V>(c, DIR * sn));
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    V b[5] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1], a[4][k1]};
+    dft_odd<5, DIR>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) v[k1 + 4 * k2] = b[k2];
+  }
+}
+
 template <int R, int DIR, class V>
 __device__ __forceinline__ void dft(V* v) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);
   else if constexpr (R == 16) dft16<DIR>(v);
+  else if constexpr (R == 20) dft20<DIR>(v);
   else dft_odd<R, DIR>(v);
 }
 

@@ -480,7 +480,7 @@ struct Solver {
   float2 *d_twf[3] = {nullptr, nullptr, nullptr}, *d_twi[3] = {nullptr, nullptr, nullptr};
   std::vector<int> plan_rad[3];    // per-axis radix plan (empty: default)
   int plan_ncol[3] = {0, 0, 0};    // per-axis workgroup columns cap (0: default)
-  int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, 256 or 512 (0: default)
+  int plan_threads[3] = {0, 0, 0}; // per-axis workgroup size, a multiple of 64 up to 512 (0: default)
   // per-axis: each XCD takes one contiguous range of tiles (1), every 8th
   // tile (0), or -1: the former when a launch has at most kXcdAutoBlocks
   // workgroups per rhs (measured: 1152^2 x pass, 72 workgroups, 34 -> 27 us;
@@ -697,6 +697,17 @@ struct Solver {
     L.fn = select_kernel(contig, mode, pro, epi, C, f64 ? kFamilyF64 : (compact ? 1 : 0));
     L.fn_generic = L.fn;
     if (!f64) L.fn = fixed_kernel(L.fn, contig, mode, pro, epi, axis, L.args.ncol, compact ? 1 : 0);
+    // radix 20 is built into plan-specialised kernels only
+    for (int q = 0; q < tab[axis].plan.nst; ++q)
+      require(tab[axis].plan.f[q].R != 20 || L.fn != L.fn_generic,
+              "radix 20 runs in plan-specialised kernels only; this launch shape has none",
+              WSIM_ERR_UNSUPPORTED);
+    // a strided tile keeps each thread on one column: the workgroup size must
+    // be a multiple of the tile width
+    require(contig || threads_of(axis, contig) % L.args.ncol == 0,
+            "workgroup size " + std::to_string(threads_of(axis, contig)) + " is not a multiple of the tile width " +
+                std::to_string(L.args.ncol),
+            WSIM_ERR_UNSUPPORTED);
     L.grid = dim3((unsigned)blocks, (unsigned)nrhs, 1);
     L.threads = threads_of(axis, contig);
     L.args.blocks = blocks;
@@ -1032,7 +1043,35 @@ struct Solver {
     rebuild_launches();
   }
 
+  // A plan the launches cannot be built for (a tile that does not fit, a
+  // radix without a kernel) leaves the previous plan in force: its tables and
+  // launches are rebuilt before the error is passed on, so no launch keeps a
+  // pointer to a freed table.
   void set_plan(const std::string& txt) {
+    std::vector<int> old_rad[3];
+    int old_ncol[3], old_thr[3], old_xcd[3];
+    for (int a = 0; a < 3; ++a) {
+      old_rad[a] = plan_rad[a];
+      old_ncol[a] = plan_ncol[a];
+      old_thr[a] = plan_threads[a];
+      old_xcd[a] = plan_xcd[a];
+    }
+    try {
+      apply_plan(txt);
+    } catch (...) {
+      for (int a = 0; a < 3; ++a) {
+        plan_rad[a] = old_rad[a];
+        plan_ncol[a] = old_ncol[a];
+        plan_threads[a] = old_thr[a];
+        plan_xcd[a] = old_xcd[a];
+        build_tables(a);
+      }
+      rebuild_launches();
+      throw;
+    }
+  }
+
+  void apply_plan(const std::string& txt) {
     size_t pos = 0;
     while (pos < txt.size()) {
       size_t end = txt.find(';', pos);
@@ -1048,7 +1087,7 @@ struct Solver {
       long long prod = 1;
       for (size_t q = 0; q < rs.size();) {
         const int x = std::atoi(rs.c_str() + q);
-        require(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 11 || x == 16,
+        require(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 11 || x == 16 || x == 20,
                 "unsupported radix in plan '" + item + "'");
         require(radix_fits(x, C), "radix " + std::to_string(x) + " is not built for " + std::to_string(C) +
                                       "-component fields in plan '" + item + "'");
@@ -1062,7 +1101,8 @@ struct Solver {
       plan_ncol[a] = slash == std::string::npos ? 0 : std::max(0, std::atoi(item.c_str() + slash + 1));
       const size_t slash2 = slash == std::string::npos ? std::string::npos : item.find('/', slash + 1);
       const int thr = slash2 == std::string::npos ? 0 : std::atoi(item.c_str() + slash2 + 1);
-      require(thr == 0 || thr == 256 || thr == 512, "plan workgroup size must be 256 or 512 in '" + item + "'");
+      require(thr == 0 || (thr >= 64 && thr <= kMaxThreads && thr % 64 == 0),
+              "plan workgroup size must be a multiple of 64 up to " + std::to_string(kMaxThreads) + " in '" + item + "'");
       plan_threads[a] = thr;
       const size_t slash3 = slash2 == std::string::npos ? std::string::npos : item.find('/', slash2 + 1);
       const std::string opt = slash3 == std::string::npos ? "" : item.substr(slash3 + 1);

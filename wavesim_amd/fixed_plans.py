@@ -165,18 +165,25 @@ def launch_shapes(N, C, plan_text):
     return out
 
 
-def shipped_shapes(path: str = os.path.join(HERE, "tuned_plans.json")):
-    """Distinct launch shapes of every shipped plan (complex64)."""
+def shipped_shapes(path: str = os.path.join(HERE, "tuned_plans.json"), extra: str = ""):
+    """Distinct launch shapes of every shipped plan (complex64), plus those of
+    `extra` plans ("NyxNxxNzxC=plan", several joined by "|"; a build-time
+    option for trying plans that need plan-specialised kernels, such as
+    radix 20)."""
     with open(path) as f:
         table = json.load(f)
-    shapes = []
+    items = []
     for key in sorted(table):
         m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", key)
-        if not m:
-            continue
-        N = [int(m.group(i)) for i in (1, 2, 3)]
-        C = int(m.group(4))
-        for s in launch_shapes(N, C, table[key]["plan"]):
+        if m:
+            items.append(([int(m.group(i)) for i in (1, 2, 3)], int(m.group(4)), table[key]["plan"]))
+    for item in filter(None, extra.split("|")):
+        grid, plan = item.split("=", 1)
+        n = [int(x) for x in grid.split("x")]
+        items.append((n[:3], n[3], plan))
+    shapes = []
+    for N, C, plan in items:
+        for s in launch_shapes(N, C, plan):
             if s not in shapes:
                 shapes.append(s)
     return shapes

@@ -43,7 +43,9 @@ _PASS = {  # (CONTIG, MODE, PRO, EPI) -> pass name
 def short_name(kernel: str) -> str:
     m = re.search(r"axis_pass_kernel<\s*(\w+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)", kernel)
     if m:
-        return _PASS.get(m.groups()[:4], "axis_pass") + f"[C={m.group(5)}]"
+        # plan-specialised kernels (FixedPlan<...>) are marked "fixed"
+        tag = ",fixed" if "FixedPlan" in kernel else ""
+        return _PASS.get(m.groups()[:4], "axis_pass") + f"[C={m.group(5)}{tag}]"
     if "control_kernel" in kernel:
         return "control"
     if "copyBuffer" in kernel:

@@ -21,7 +21,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-from . import plans
+from . import capi, plans
 from .solver import MBSSolver
 
 ALPHA = 0.7  # f_mix.m:3
@@ -309,7 +309,14 @@ class WaveSimBase:
             self._solver_nrhs = nrhs
             plan = plans.lookup(self.N, nrhs, len(self.wiggles))
             if plan:
-                self._solver.plan = plan
+                try:
+                    self._solver.plan = plan
+                except capi.WsimError as e:
+                    # a plan that needs kernels this library does not run
+                    # (plan-specialised kernels turned off): the library keeps
+                    # the plan it had, which computes the same field
+                    if e.code != capi.WSIM_ERR_UNSUPPORTED:
+                        raise
         return self._solver
 
     def adopt_solver(self, other: "WaveSimBase"):
