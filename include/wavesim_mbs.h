@@ -196,8 +196,9 @@ int wsim_solver_profile(wsim_solver* s, int reps, int max_kernels, double* ms, d
  * 512 (strided axes: also a multiple of the workgroup columns), 0 = default /
  * optional "xcd": each of the 8 XCDs takes one contiguous range of the
  * axis's tiles instead of every 8th tile).  Radices are 2, 3, 4, 5, 7, 8,
- * 11, 16 (16 not for 3-component fields), and 20 in launch shapes that have
- * a plan-specialised kernel (wsim_solver_fixed_launches).  A plan changes
+ * 11, 16 (16 not for 3-component fields), and, for scalar fields in launch
+ * shapes that have a plan-specialised kernel (wsim_solver_fixed_launches),
+ * the composite register radices 9, 12, 20, 24, 32, 36.  A plan changes
  * speed only.  A plan whose launches cannot be built returns an error and
  * leaves the previous plan in force.
  * wsim_solver_tune times whole iterations of this problem for candidate radix

@@ -238,6 +238,93 @@ __device__ __forceinline__ void dft20(V* v) {
   }
 }
 
+// cos and sin of 2 pi m / R as compile-time constants (Taylor series on the
+// octant-reduced angle, double precision)
+constexpr double kPi = 3.14159265358979323846;
+constexpr double taylor_sin(double x) {
+  double t = x, s = x;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+// (cos, sin)(2 pi m / R) from the angle's octant: m/R in eighths of a turn
+constexpr double unit_cos(int m, int R) {
+  m = ((m % R) + R) % R;
+  const int o = (8 * m) / R;          // octant 0..7
+  const double r = kPi * (double)(8 * m - o * R) / (4.0 * R);  // 0 <= r < pi/4
+  switch (o) {
+    case 0: return taylor_cos(r);
+    case 1: return taylor_sin(kPi / 4 - r);
+    case 2: return -taylor_sin(r);
+    case 3: return -taylor_cos(kPi / 4 - r);
+    case 4: return -taylor_cos(r);
+    case 5: return -taylor_sin(kPi / 4 - r);
+    case 6: return taylor_sin(r);
+    default: return taylor_cos(kPi / 4 - r);
+  }
+}
+constexpr double unit_sin(int m, int R) {
+  m = ((m % R) + R) % R;
+  const int o = (8 * m) / R;
+  const double r = kPi * (double)(8 * m - o * R) / (4.0 * R);
+  switch (o) {
+    case 0: return taylor_sin(r);
+    case 1: return taylor_cos(kPi / 4 - r);
+    case 2: return taylor_cos(r);
+    case 3: return taylor_sin(kPi / 4 - r);
+    case 4: return -taylor_sin(r);
+    case 5: return -taylor_cos(kPi / 4 - r);
+    case 6: return -taylor_cos(r);
+    default: return -taylor_sin(kPi / 4 - r);
+  }
+}
+
+template <int R, int DIR, class V>
+__device__ __forceinline__ void dft(V* v);
+
+// radix R = R1 x R2 in registers (Cooley-Tukey, n = R2 n1 + n2, k = k1 + R1 k2):
+// R1-point DFTs over n1, twiddles w_R^(n2 k1), R2-point DFTs over n2.  Used
+// for the composite radices of plan-specialised kernels (9, 12, 24, 32, 36),
+// which shorten long axes to two or three stages.
+template <int R1, int R2, int DIR, class V>
+__device__ __forceinline__ void dft_ct(V* v) {
+  using S = real_of<V>;
+  constexpr int R = R1 * R2;
+  V a[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
+    dft<R1, DIR>(a[n2]);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) {
+      const S c = S(unit_cos(n2 * k1, R)), sn = S(unit_sin(n2 * k1, R));
+      a[n2][k1] = cmul(a[n2][k1], mkv<V>(c, DIR * sn));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    V b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+    dft<R2, DIR>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
 template <int R, int DIR, class V>
 __device__ __forceinline__ void dft(V* v) {
   if constexpr (R == 2) dft2<DIR>(v);
@@ -245,6 +332,11 @@ __device__ __forceinline__ void dft(V* v) {
   else if constexpr (R == 8) dft8<DIR>(v);
   else if constexpr (R == 16) dft16<DIR>(v);
   else if constexpr (R == 20) dft20<DIR>(v);
+  else if constexpr (R == 9) dft_ct<3, 3, DIR>(v);
+  else if constexpr (R == 12) dft_ct<4, 3, DIR>(v);
+  else if constexpr (R == 24) dft_ct<8, 3, DIR>(v);
+  else if constexpr (R == 32) dft_ct<4, 8, DIR>(v);
+  else if constexpr (R == 36) dft_ct<4, 9, DIR>(v);
   else dft_odd<R, DIR>(v);
 }
 

@@ -415,6 +415,13 @@ static int contig_pitch(int n) { return n + (n >> WSIM_PAD_SHIFT) + 1; }
 
 static int its_for(int C, int R, int family = 0) { return its_per_stage(ept_for(C, family), C, R); }
 
+// radices of the run-time-plan kernels, and the composite register radices
+// (R1 x R2 in registers) that only plan-specialised kernels run (scalar fields)
+static bool radix_generic(int R) {
+  return R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 16;
+}
+static bool radix_specialised(int R) { return R == 9 || R == 12 || R == 20 || R == 24 || R == 32 || R == 36; }
+
 // every stage of the plan covers ncol lines of length n within the
 // butterflies-per-thread budget of the kernel family
 static bool tile_fits(const AxisPlan& pl, int n, int ncol, int C, int family, int NT) {
@@ -697,10 +704,11 @@ struct Solver {
     L.fn = select_kernel(contig, mode, pro, epi, C, f64 ? kFamilyF64 : (compact ? 1 : 0));
     L.fn_generic = L.fn;
     if (!f64) L.fn = fixed_kernel(L.fn, contig, mode, pro, epi, axis, L.args.ncol, compact ? 1 : 0);
-    // radix 20 is built into plan-specialised kernels only
+    // the composite radices are built into plan-specialised kernels only
     for (int q = 0; q < tab[axis].plan.nst; ++q)
-      require(tab[axis].plan.f[q].R != 20 || L.fn != L.fn_generic,
-              "radix 20 runs in plan-specialised kernels only; this launch shape has none",
+      require(radix_generic(tab[axis].plan.f[q].R) || L.fn != L.fn_generic,
+              "radix " + std::to_string(tab[axis].plan.f[q].R) +
+                  " runs in plan-specialised kernels only; this launch shape has none",
               WSIM_ERR_UNSUPPORTED);
     // a strided tile keeps each thread on one column: the workgroup size must
     // be a multiple of the tile width
@@ -1087,10 +1095,10 @@ struct Solver {
       long long prod = 1;
       for (size_t q = 0; q < rs.size();) {
         const int x = std::atoi(rs.c_str() + q);
-        require(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 11 || x == 16 || x == 20,
-                "unsupported radix in plan '" + item + "'");
-        require(radix_fits(x, C), "radix " + std::to_string(x) + " is not built for " + std::to_string(C) +
-                                      "-component fields in plan '" + item + "'");
+        require(radix_generic(x) || radix_specialised(x), "unsupported radix in plan '" + item + "'");
+        require(radix_specialised(x) ? C == 1 : radix_fits(x, C),
+                "radix " + std::to_string(x) + " is not built for " + std::to_string(C) +
+                    "-component fields in plan '" + item + "'");
         rad.push_back(x);
         prod *= x;
         while (q < rs.size() && rs[q] != ',') ++q;

@@ -28,6 +28,7 @@ PAD_SHIFT = 3
 MAX_ITS = 6
 # pass variants as (contig, mode, prologue, epilogue): mode 0 fwd, 1 inv, 2 mid
 FWD_STRIDED, INV_STRIDED, MID_STRIDED, FUSED_FIRST, MIX_FIRST = (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0), (1, 1, 0, 3), (1, 1, 0, 1)
+FIRST_FWD = (1, 0, 1, 0)  # the first forward pass (source, ramps): once per solve when the last pass is fused
 
 
 def ept_for(C: int, family: int) -> int:
@@ -158,6 +159,7 @@ def launch_shapes(N, C, plan_text):
 
     first = axes[0]
     add(FUSED_FIRST if len(plan[first][0]) >= 2 else MIX_FIRST, first)
+    add(FIRST_FWD, first)
     for a in axes[1:-1]:
         add(FWD_STRIDED, a)
         add(INV_STRIDED, a)
