@@ -265,6 +265,7 @@ __device__ void control_one(DevState* st, const double* partials, int nblk) {
     st->calc_energy = ((it - 1) % st->interval) == 0;
     st->it = it + 1;
     st->all_stopped = all;
+    if (st->host_stopped != nullptr && all) *st->host_stopped = 1;
   }
 }
 
@@ -505,6 +506,8 @@ struct Solver {
   hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
   DevState host_state{};
+  int* h_stopped = nullptr;  // pinned host flag the control step sets when every rhs has stopped
+  hipEvent_t batch_ev[2] = {nullptr, nullptr};  // ends of the two batches in flight
   long long last_iters = 0;
   // Allocation-free mode (wsim_run_algorithm_noalloc): every device buffer is
   // carved from a caller-provided region (arena), gamma and E' are the
@@ -530,6 +533,10 @@ struct Solver {
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
+    if (h_stopped) (void)hipHostFree(h_stopped);
+    h_stopped = nullptr;
+    for (hipEvent_t& e : batch_ev)
+      if (e) (void)hipEventDestroy(e);
     void* ptrs[] = {dE, T, S, probe, partials, dst, dprobe, d_stopped, d_conv, d_niter,
                     d_last, d_diff, d_eall, p_stopped, p_dummy, d_args, d_probe_args, d_ref, W0, d_pre};
     for (void* p : ptrs) dfree(p);
@@ -1588,6 +1595,14 @@ struct Solver {
     }
     h.all_stopped = all;
     std::vector<double> inf(nrhs, INFINITY);
+    if (arena == nullptr && !pstd) {
+      if (h_stopped == nullptr)
+        HIPCHK(hipHostMalloc((void**)&h_stopped, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+      *h_stopped = all;
+      int* dev_flag = nullptr;
+      HIPCHK(hipHostGetDevicePointer((void**)&dev_flag, h_stopped, 0));
+      h.host_stopped = dev_flag;
+    }
     HIPCHK(hipMemcpyAsync(dst, &h, sizeof(h), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_stopped, st.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_conv, zero.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1633,20 +1648,43 @@ struct Solver {
       long long d = std::min<long long>(ci, 256);
       while (ci % d != 0) --d;
       batch = d;
-    } else if (batch <= 0) {
-      // about 2 ms of work per batch, a multiple of Nw
-      const double it_us = 112.0 * (double)nvox * C * nrhs / 5e6;  // ~5 TB/s
-      long long k = (long long)std::ceil(2000.0 / std::max(it_us, 1.0));
+    }
+    const bool use_graph = std::getenv("WSIM_NO_GRAPH") == nullptr && arena == nullptr;
+    // Without a callback two batches are kept in flight: the host polls the
+    // stop flag the control step writes into pinned host memory while the
+    // next batch runs, so the GPU does not wait for the host between batches,
+    // and a batch needs only cover the host's turn-around (about 150 us of
+    // work, a multiple of Nw).  At most two batches of no-op launches follow
+    // the stop.  With a callback (or without graphs) a batch ends in a host
+    // synchronisation; it then holds about 2 ms of work.
+    const bool pipelined = use_graph && cb == nullptr && h_stopped != nullptr;
+    if (batch <= 0) {
+      const double it_us = std::max(112.0 * (double)nvox * C * nrhs / 5e6, 20.0);  // ~5 TB/s, >= 20 us
+      long long k = (long long)std::ceil((pipelined ? 150.0 : 2000.0) / it_us);
       k = std::max<long long>(nw, std::min<long long>(k, 256));
       batch = ((k + nw - 1) / nw) * nw;
     }
-    const bool use_graph = std::getenv("WSIM_NO_GRAPH") == nullptr && arena == nullptr;
     if (use_graph) capture_graph((int)batch);
+    if (pipelined) {
+      for (hipEvent_t& e : batch_ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if (!*h_stopped) {
+        HIPCHK(hipGraphLaunch(gexec, stream));
+        HIPCHK(hipEventRecord(batch_ev[0], stream));
+        for (long long b = 1;; ++b) {
+          HIPCHK(hipGraphLaunch(gexec, stream));
+          HIPCHK(hipEventRecord(batch_ev[b & 1], stream));
+          HIPCHK(hipEventSynchronize(batch_ev[(b - 1) & 1]));
+          if (__atomic_load_n(h_stopped, __ATOMIC_ACQUIRE)) break;
+        }
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     long long done = 0;
     double next_cb = cbint > 0 ? cbint : INFINITY;
     std::vector<double> hist;
     int rc = WSIM_OK;
-    for (;;) {
+    for (; !pipelined;) {
       HIPCHK(hipMemcpy(&host_state, dst, sizeof(DevState), hipMemcpyDeviceToHost));
       if (host_state.all_stopped) break;
       if (use_graph)

@@ -156,6 +156,10 @@ struct DevState {
   double amp_re, amp_im;
   float2 amp, phase;
   double ipc;
+  // host-visible copy of all_stopped (pinned, mapped host memory; null: none),
+  // written by the control step so that the host can poll it while the next
+  // batch of iterations is already queued
+  int* host_stopped;
 };
 
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
