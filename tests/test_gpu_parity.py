@@ -599,3 +599,24 @@ def test_failure_exit_on_epsilon_below_the_bound(gpu, builder, vector):
     r = g.solver().result(0)
     print(f"{builder.__name__}: stopped after {r.iterations} iterations (rule on the oracle: {want})", flush=True)
     assert r.diverged and want is not None and abs(r.iterations - want) <= nw
+
+
+@pytest.mark.parametrize("builder", [cases.case_2d_disordered, cases.case_3d_disordered])
+def test_control_once_per_wiggle_cycle_equals_every_iteration(gpu, builder, monkeypatch):
+    """Solves run the control step (energy reduction, histories, stop rule)
+    once per wiggle cycle, iteration j of a cycle writing its energy partials
+    into slot j; WSIM_CYCLE_CONTROL=0 runs it after every iteration.  Same
+    iterations, histories and field, bit for bit."""
+    import wavesim_amd as w
+    n, opt, src = builder()
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WSIM_CYCLE_CONTROL", mode)
+        g = w.WaveSim(n, opt)
+        E, st = g.exec(cases.gpu_source(src))
+        out.append((E, st))
+    (E1, s1), (E0, s0) = out
+    assert s1.iterations == s0.iterations
+    assert np.array_equal(s1.diff_energy, s0.diff_energy)
+    assert np.array_equal(s1.energy_every, s0.energy_every)
+    assert np.array_equal(E1, E0)

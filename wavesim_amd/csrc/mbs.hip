@@ -197,15 +197,26 @@ static const ShippedPlan kShippedPlans[] = {
 #include "shipped_plans.inc"
 #endif
     {0, 0, 0, 0, 0, 0, nullptr}};
+// Decisions every thread takes (return, skip a stopped rhs) come from one
+// thread's read of the state, broadcast through LDS: the state is written by
+// thread 0 of this workgroup, possibly in the previous step of the same launch
+// (control_cycle_kernel), and all threads must agree on the barriers they
+// reach.
 __device__ void control_one(DevState* st, const double* partials, int nblk) {
   constexpr int NW = kControlThreads / 64;
   __shared__ double red[NW];
-  if (st->all_stopped) return;
-  const int it = st->it;
+  __shared__ int flag;
+  if (threadIdx.x == 0) flag = st->all_stopped;
+  __syncthreads();
+  if (flag) return;
+  const int it = st->it;  // used by thread 0 only
   const int nrhs = st->nrhs;
   int all = 1;
   for (int r = 0; r < nrhs; ++r) {
-    if (st->stopped[r]) continue;
+    __syncthreads();  // every thread has read the previous rhs's flag
+    if (threadIdx.x == 0) flag = st->stopped[r];
+    __syncthreads();
+    if (flag) continue;
     // four independent accumulators keep four loads in flight per thread;
     // the summation order is fixed, so the result is deterministic
     const double* pr = partials + (size_t)r * nblk;
@@ -271,6 +282,19 @@ __device__ void control_one(DevState* st, const double* partials, int nblk) {
 
 __global__ void __launch_bounds__(kControlThreads) control_kernel(DevState* st, const double* partials, int nblk) {
   control_one(st, partials, nblk);
+}
+
+// The control steps of L iterations at once, in order (iteration j's energy
+// partials in slot j, `slot` doubles apart).  A solve runs one per wiggle
+// cycle: the stop test applies at multiples of Nw only (Simulation.m:202), so
+// no rhs stops inside a cycle except on the failure exit (whose iteration is
+// still the one recorded)
+__global__ void __launch_bounds__(kControlThreads) control_cycle_kernel(DevState* st, const double* partials, int nblk,
+                                                                        int L, long long slot) {
+  for (int j = 0; j < L; ++j) {
+    control_one(st, partials + (size_t)j * slot, nblk);
+    __syncthreads();
+  }
 }
 
 // grouped launches: workgroup e runs the control step of problem e and
@@ -497,7 +521,8 @@ struct Solver {
   std::vector<Launch> iter;  // one MBS iteration
   Launch pre{};                // first forward pass of iteration 1 when fused (fn == nullptr: none)
   PassArgs* d_pre = nullptr;
-  PassArgs* d_args = nullptr;  // device copies of iter[k].args
+  PassArgs* d_args = nullptr;  // device copies of iter[k].args, then those of the other iterations of a cycle
+  int part_cap = 0;            // energy partials allocated per iteration slot (of nw slots)
   std::vector<Launch> probe_iter;
   PassArgs* d_probe_args = nullptr;
   size_t lds_limit = 64 * 1024;    // per-workgroup LDS the tile search aims at
@@ -973,17 +998,18 @@ struct Solver {
     std::vector<Launch> it2;
     build_iteration(it2, false);
     const int nb = (int)it2.back().grid.x;
-    if (!partials || nb > nblk) {
+    if (!partials || nb > part_cap) {
       dfree(partials);
       partials = nullptr;
-      dalloc(&partials, (size_t)nb * nrhs);
+      dalloc(&partials, (size_t)nb * nrhs * nw);  // one slot per iteration of a wiggle cycle
+      part_cap = nb;
     }
     nblk = nb;
     for (auto& L : it2) L.args.partials = partials;
     dfree(d_args);
     d_args = nullptr;
     iter.swap(it2);
-    upload_args(iter, &d_args);
+    upload_cycle_args();
     dfree(d_pre);
     d_pre = nullptr;
     if (pre.fn) {
@@ -1214,6 +1240,24 @@ struct Solver {
     return plan_string();
   }
 
+  // the launches of the nw iterations of a wiggle cycle: iteration j's copy of
+  // every pass carries it_off = j and energy-partials slot j; iter[k].dargs is
+  // iteration 0's (per-iteration control, groups, profiling)
+  void upload_cycle_args() {
+    const size_t n = iter.size();
+    std::vector<PassArgs> h(n * nw);
+    for (int j = 0; j < nw; ++j)
+      for (size_t k = 0; k < n; ++k) {
+        PassArgs a = iter[k].args;
+        a.it_off = j;
+        if (a.partials) a.partials = partials + (size_t)j * nrhs * nblk;
+        h[j * n + k] = a;
+      }
+    dalloc(&d_args, h.size());
+    HIPCHK(hipMemcpy(d_args, h.data(), h.size() * sizeof(PassArgs), hipMemcpyHostToDevice));
+    for (size_t k = 0; k < n; ++k) iter[k].dargs = d_args + k;
+  }
+
   void upload_args(std::vector<Launch>& ls, PassArgs** dptr) {
     std::vector<PassArgs> h(ls.size());
     for (size_t k = 0; k < ls.size(); ++k) h[k] = ls[k].args;
@@ -1230,10 +1274,29 @@ struct Solver {
     HIPCHK(hipGetLastError());
   }
 
+  // k iterations; whole wiggle cycles run their control steps once per cycle
+  // (WSIM_CYCLE_CONTROL=0: every iteration)
   void enqueue_iterations(int k) {
-    for (int i = 0; i < k; ++i) {
-      for (const auto& L : iter) launch(L);
-      launch_control();
+    const char* e = std::getenv("WSIM_CYCLE_CONTROL");
+    const bool cyc = !pstd && nw > 1 && k % nw == 0 && !(e && std::atoi(e) == 0);
+    if (!cyc) {
+      for (int i = 0; i < k; ++i) {
+        for (const auto& L : iter) launch(L);
+        launch_control();
+      }
+      return;
+    }
+    const size_t n = iter.size();
+    for (int c = 0; c < k / nw; ++c) {
+      for (int j = 0; j < nw; ++j)
+        for (size_t q = 0; q < n; ++q) {
+          const Launch& L = iter[q];
+          hipLaunchKernelGGL(L.fn, L.grid, dim3(L.threads), L.lds, stream, (const PassArgs*)(d_args + j * n + q));
+          HIPCHK(hipGetLastError());
+        }
+      hipLaunchKernelGGL(control_cycle_kernel, dim3(1), dim3(kControlThreads), 0, stream, dst, partials, nblk, nw,
+                         (long long)nrhs * nblk);
+      HIPCHK(hipGetLastError());
     }
   }
 

@@ -192,6 +192,11 @@ struct PassArgs {
   const float2* gamma;
   double* partials;  // [rhs][blocks]
   const DevState* st;
+  // iteration of this launch relative to the state's counter: solves advance
+  // the counter once per wiggle cycle (one control step for Nw iterations),
+  // and the launches of iteration j of a cycle carry it_off = j and their own
+  // energy-partials slot
+  int it_off;
   const float* pe[3];     // [nw][N[a]] scaled Fourier coordinates
   const float2* ramp[3];  // [nw][N[a]] wiggle phase ramps
   float2 k02e;
@@ -1004,7 +1009,7 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
   L2 twi_l = (L2)(smem + off_twi);
   L2 lds = (L2)(smem + off_data);
   const int nthr = (int)blockDim.x;  // 256 or 512 (<= NT), chosen per pass by the plan
-  const int it_cur = st->it;
+  const int it_cur = st->it + a.it_off;
   const int nw = st->nw;
   const int w = (it_cur - 1) % nw;
   const int w2 = it_cur % nw;  // the next iteration's wiggle (EPI_MIX_FWD)
