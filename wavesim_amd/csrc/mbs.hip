@@ -755,6 +755,14 @@ struct Solver {
     L.args.nrhs = nrhs;
     L.args.grp_n = 1;
     L.args.grp_end = nullptr;
+    // several rhs: the mix passes (the ones that read gamma) run a tile's rhs
+    // back to back on one XCD (WSIM_RHS_INNER=0: rhs along y)
+    const char* env_ri = std::getenv("WSIM_RHS_INNER");
+    if (nrhs > 1 && (epi == EPI_MIX || epi == EPI_MIX_FWD) && !(env_ri && std::atoi(env_ri) == 0)) {
+      L.args.rhs_inner = 1;
+      L.grid = dim3((unsigned)(8ll * ((blocks + 7) / 8) * nrhs), 1, 1);
+      require(L.grid.x < (1u << 31), "too many workgroups for one launch");
+    }
     L.name = name;
     // algorithmic bytes per voxel, averaged over an even and an odd
     // iteration (E' is read and written on even iterations only): the mix
@@ -925,6 +933,10 @@ struct Solver {
       for (auto& L : out) {
         L.args.st = dprobe;
         L.grid.y = 1;
+        if (L.args.rhs_inner) {  // the probe runs one rhs
+          L.args.rhs_inner = 0;
+          L.grid.x = (unsigned)L.args.blocks;
+        }
       }
   }
 
@@ -997,7 +1009,7 @@ struct Solver {
   void rebuild_launches() {
     std::vector<Launch> it2;
     build_iteration(it2, false);
-    const int nb = (int)it2.back().grid.x;
+    const int nb = it2.back().args.blocks;  // the mix pass's tiles (its grid also spans the rhs when rhs_inner)
     if (!partials || nb > part_cap) {
       dfree(partials);
       partials = nullptr;

@@ -219,6 +219,11 @@ struct PassArgs {
   int nrhs;            // this entry's rhs count; workgroups with y beyond it idle
   int reverse;         // walk the tiles from the end of memory
   int xcd;             // hand each XCD one contiguous range of tiles
+  // several rhs, mix passes: the rhs index varies fastest along x (grid
+  // 8 * ceil(blocks / 8) * nrhs by 1), so the workgroups of one tile's rhs
+  // run on one XCD one after another and read the shared gamma tile from its
+  // L2 instead of HBM
+  int rhs_inner;
   // PSTD (PSTD.m:79-127): the two time levels E, E_prev alternate between
   // fld[0] and fld[1] (step k reads fld[(k-1)&1], writes fld[k&1]); the
   // middle pass multiplies by the Laplacian -|p|^2 (pe = p, unscaled)
@@ -977,7 +982,22 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
     // the host launches a plan-specialised kernel only for its own shape
     if (a.n != K::kN || a.ncol != K::kNcol || (int)blockDim.x != K::kNT || a.plan.nst != K::S) return;
   }
-  if (a.xcd) {
+  int rhs = blockIdx.y;
+  if (a.rhs_inner) {
+    // XCD x runs workgroups x, x+8, x+16, ... in order: its k-th takes rhs
+    // k mod nrhs of its (k / nrhs)-th tile
+    if (blockIdx.y != 0) return;
+    const int x = bx & 7, k = bx >> 3, j = k / a.nrhs;
+    rhs = k - j * a.nrhs;
+    if (a.xcd) {
+      const int q = a.blocks >> 3, r = a.blocks & 7;
+      if (j >= q + (x < r ? 1 : 0)) return;
+      bx = x * q + (x < r ? x : r) + j;
+    } else {
+      bx = j * 8 + x;
+      if (bx >= a.blocks) return;
+    }
+  } else if (a.xcd) {
     // workgroups go to the 8 XCDs round-robin by index: give XCD x the
     // contiguous block range [x*q + min(x, r), +q + (x < r)) (a bijection)
     const int q = a.blocks >> 3, r = a.blocks & 7, x = bx & 7, j = bx >> 3;
@@ -985,7 +1005,6 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
   }
   if (a.reverse) bx = a.blocks - 1 - bx;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int rhs = blockIdx.y;
   if (rhs >= a.nrhs) return;
   const DevState* st = a.st;
   if (st->stopped[rhs]) return;
