@@ -62,3 +62,32 @@ def test_fixed_kernels_match_run_time_plan_kernels(gpu, name, monkeypatch):
     assert rf.iterations == rg.iterations
     assert rel_l2(Ef, Eg) <= tol
     assert np.allclose(rf.diff_energy, rg.diff_energy, rtol=1e-5)
+
+
+def test_complex128_fixed_kernels_match_run_time_plan_kernels(gpu, monkeypatch):
+    """complex128 solves on a shipped plan's shape (the 256^2 homogeneous
+    workload, 320^2 grid) run plan-specialised double kernels: the same
+    iterations and the field of the run-time-plan double kernels to 1e-12."""
+    import torch
+    from wavesim_amd import WaveSim
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, source = WORKLOADS["homogeneous_2d_256"]()
+    opt = dict(opt, single_precision=False)
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WSIM_FIXED", mode)
+        sim = WaveSim(nmap, opt)
+        sv = sim.solver(1)
+        src = sim._prepare(source)
+        sv.set_source(0, src.positions, src.values)
+        nf = sv.fixed_launches
+        sv.solve(sim.max_iterations(), sim.energy_threshold)
+        torch.cuda.synchronize()
+        out.append((nf, sv.result(0).iterations, sv.field(0)))
+        sv.close()
+    (nf1, it1, E1), (nf0, it0, E0) = out
+    print(f"complex128 320^2: fixed launches {nf1}, iterations {it1} / {it0}, rel-L2 {rel_l2(E1, E0):.2e}", flush=True)
+    assert E1.dtype == np.complex128
+    assert nf1 == 2 and nf0 == 0
+    assert it1 == it0
+    assert rel_l2(E1, E0) <= 1e-12

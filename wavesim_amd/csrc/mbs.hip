@@ -735,7 +735,7 @@ struct Solver {
                          tile_fits(tab[axis].plan, N[axis], L.args.ncol, C, 1, threads_of(axis, contig));
     L.fn = select_kernel(contig, mode, pro, epi, C, f64 ? kFamilyF64 : (compact ? 1 : 0));
     L.fn_generic = L.fn;
-    if (!f64) L.fn = fixed_kernel(L.fn, contig, mode, pro, epi, axis, L.args.ncol, compact ? 1 : 0);
+    L.fn = fixed_kernel(L.fn, contig, mode, pro, epi, axis, L.args.ncol, f64 ? kFamilyF64 : (compact ? 1 : 0));
     // the composite radices are built into plan-specialised kernels only
     for (int q = 0; q < tab[axis].plan.nst; ++q)
       require(radix_generic(tab[axis].plan.f[q].R) || L.fn != L.fn_generic,

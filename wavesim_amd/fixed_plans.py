@@ -60,8 +60,11 @@ def r16(x: int) -> int:
     return (x + 15) & ~15
 
 
-def lds_header(ncol, n, ntw, fuse) -> int:
-    return r16(ncol * 56) + (2 if fuse else 1) * r16(n * 8) + r16(n * 4) + ntw * 8
+def lds_header(ncol, n, ntw, fuse, f64=False) -> int:
+    """Column table, ramps, coordinates and twiddles in front of the tile (complex64:
+    56-byte column entries, 8-byte complex, 4-byte real; complex128: 96, 16, 8)."""
+    ci, cz, rz = (96, 16, 8) if f64 else (56, 8, 4)
+    return r16(ncol * ci) + (2 if fuse else 1) * r16(n * cz) + r16(n * rz) + ntw * cz
 
 
 def contig_pitch(n: int) -> int:
@@ -81,8 +84,11 @@ def parse_plan(text: str):
     return out
 
 
-def configure(N, C, rad, ncap, thr, axis, contig, mode, fuse):
-    """(ncol, threads, family) of one pass launch, as the host chooses them."""
+def configure(N, C, rad, ncap, thr, axis, contig, mode, fuse, f64=False):
+    """(ncol, threads, family, waves, early) of one pass launch, as the host
+    chooses them (family 0/1: complex64 kernels, 2: complex128)."""
+    cz = 16 if f64 else 8
+    base = 2 if f64 else 0
     n = N[axis]
     nthr = thr if thr > 0 else (256 if contig else 512)
     ntwf, ntwi = twiddles(rad), twiddles(rad[::-1])
@@ -93,10 +99,10 @@ def configure(N, C, rad, ncap, thr, axis, contig, mode, fuse):
     nvox = N[0] * N[1] * N[2]
 
     def fits(c, cap):
-        if not tile_fits(rad, n, c, C, 0, nthr):
+        if not tile_fits(rad, n, c, C, base, nthr):
             return False
-        b = C * c * contig_pitch(n) * 8 if contig else n * (C * c + 1) * 8
-        return b + lds_header(c, n, ntw, fuse) <= cap
+        b = C * c * contig_pitch(n) * cz if contig else n * (C * c + 1) * cz
+        return b + lds_header(c, n, ntw, fuse, f64) <= cap
 
     cap_b = LDS_TARGET if fits(1, LDS_TARGET) else LDS_DEVICE
     cap = ncap if ncap > 0 else (16 if contig else 32)
@@ -121,13 +127,14 @@ def configure(N, C, rad, ncap, thr, axis, contig, mode, fuse):
             ncol >>= 1
     if ncol == 0:
         return None
-    family = 1 if tile_fits(rad, n, ncol, C, 1, nthr) else 0
-    b = C * ncol * contig_pitch(n) * 8 if contig else n * (C * ncol + 1) * 8
-    lds = b + lds_header(ncol, n, ntw, fuse)
+    family = 2 if f64 else (1 if tile_fits(rad, n, ncol, C, 1, nthr) else 0)
+    b = C * ncol * contig_pitch(n) * cz if contig else n * (C * ncol + 1) * cz
+    lds = b + lds_header(ncol, n, ntw, fuse, f64)
     # waves per SIMD that the LDS allows: the kernel's register count must
     # not be what holds occupancy below it (4 SIMDs; at most 6 asked, so
-    # that no kernel is squeezed below 80 registers and spills)
-    waves = min(6, max(1, (LDS_DEVICE // lds) * (nthr // 64) // 4))
+    # that no kernel is squeezed below 80 registers and spills; complex128
+    # kernels, whose values take twice the registers, at most 2)
+    waves = min(2 if f64 else 6, max(1, (LDS_DEVICE // lds) * (nthr // 64) // 4))
     # first-stage loads before the table set-up where at most two workgroups
     # share a CU (measured on the 320^3 passes: the fused pass with two
     # workgroups per CU 326 -> 310 us, the strided passes with three slower)
@@ -135,7 +142,7 @@ def configure(N, C, rad, ncap, thr, axis, contig, mode, fuse):
     return ncol, nthr, family, waves, early
 
 
-def launch_shapes(N, C, plan_text):
+def launch_shapes(N, C, plan_text, f64=False):
     """The per-iteration pass launches of a complex64 solve on grid N with the
     given plan: [(contig, mode, pro, epi, C, family, n, ncol, threads, waves per SIMD, early loads,
     radices)]."""
@@ -148,7 +155,7 @@ def launch_shapes(N, C, plan_text):
     def add(variant, axis):
         rad, ncap, thr = plan[axis]
         contig, mode, pro, epi = variant
-        cfg = configure(N, C, rad, ncap, thr, axis, bool(contig), mode, epi == 3)
+        cfg = configure(N, C, rad, ncap, thr, axis, bool(contig), mode, epi == 3, f64)
         if cfg is None:
             return
         ncol, nthr, family, waves, early = cfg
@@ -184,10 +191,11 @@ def shipped_shapes(path: str = os.path.join(HERE, "tuned_plans.json"), extra: st
         n = [int(x) for x in grid.split("x")]
         items.append((n[:3], n[3], plan))
     shapes = []
-    for N, C, plan in items:
-        for s in launch_shapes(N, C, plan):
-            if s not in shapes:
-                shapes.append(s)
+    for f64 in (False, True):  # complex64, then the complex128 kernels of the same plans
+        for N, C, plan in items:
+            for s in launch_shapes(N, C, plan, f64):
+                if s not in shapes:
+                    shapes.append(s)
     return shapes
 
 
@@ -195,10 +203,11 @@ def kernel_unit(i: int, shape) -> str:
     """C++ source of plan-specialised kernel number i (one compilation unit)."""
     ct, m, p, e, C, fam, n, ncol, nthr, waves, early, rad = shape
     rs = ", ".join(str(r) for r in rad)
+    F = "double" if fam == 2 else "float"
     return ("// generated by wavesim_amd/build.py from tuned_plans.json (fixed_plans.py)\n"
             '#include "mbs_kernels.hpp"\n#include "mbs_config.hpp"\n\nnamespace wsim {\n'
             f"KernelFn wsim_fixed_{i}() {{ return axis_pass_kernel<{'true' if ct else 'false'}, {m}, {p}, {e}, {C}, "
-            f"{nthr}, {ept_for(C, fam)}, float, FixedPlan<{n}, {ncol}, {nthr}, {waves}, {'true' if early else 'false'}, {rs}>>; }}\n"
+            f"{nthr}, {ept_for(C, fam)}, {F}, FixedPlan<{n}, {ncol}, {nthr}, {waves}, {'true' if early else 'false'}, {rs}>>; }}\n"
             "}  // namespace wsim\n")
 
 
