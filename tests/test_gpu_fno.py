@@ -46,6 +46,22 @@ def test_gemm_matches_numpy(gpu, M, N, K, transA, transB, nb):
     assert np.allclose(out, ref, rtol=1e-5, atol=1e-4 * np.abs(ref).max())
 
 
+def test_gemm_without_unit_strides_matches_numpy(gpu):
+    """Operands whose rows and columns are both strided (every second
+    element): the staging reads them element by element."""
+    from oracle.fno import round_bf16
+    from wavesim_amd.learned import kernels as Kn
+    rng = np.random.default_rng(3)
+    M, N, K = 70, 45, 50
+    A = rng.standard_normal((M, 2 * K)).astype(np.float32)
+    B = rng.standard_normal((2 * K, 2 * N)).astype(np.float32)
+    Cm = np.zeros((M, N), np.float32)
+    dA, dB, dC = _t(A), _t(B), _t(Cm)
+    Kn.gemm(M, N, K, dA, (2 * K, 2), dB, (2 * 2 * N, 2), dC, (N, 1))
+    ref = round_bf16(A[:, ::2]) @ round_bf16(B[::2, ::2])
+    assert np.allclose(dC.cpu().numpy(), ref, rtol=1e-5, atol=1e-4 * np.abs(ref).max())
+
+
 def _pair(seed=0):
     from oracle import fno as O
     from wavesim_amd.learned import fno as G

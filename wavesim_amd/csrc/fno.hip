@@ -2,8 +2,9 @@
 //
 // Everything the network needs, forward and backward, is one of:
 //   * wfno_gemm: strided, two-level-batched real GEMM  C = alpha A B + beta C
-//     on BF16 matrix cores (v_mfma_f32_32x32x8_bf16, FP32 accumulate); A and B
-//     are FP32 in memory with arbitrary strides and are rounded to BF16
+//     on BF16 matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulate); A and B
+//     are FP32 in memory with arbitrary strides, read four consecutive
+//     elements at a time along their unit-stride dimension and rounded to BF16
 //     (round-to-nearest-even) when staged into LDS.  Transposes are stride
 //     swaps, so the backward GEMMs (dA = dC B^T, dB = A^T dC) use the same kernel.
 //     The truncated DFTs of the spectral layers are GEMMs with cos/sin matrices.
@@ -20,7 +21,6 @@
 
 namespace wfno {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 static thread_local std::string g_err;
@@ -32,12 +32,77 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
-constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;  // LDS row pitch in bf16 (80 B)
+constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;  // LDS row pitch in bf16 (80 B: 16-byte aligned rows)
 
-// Workgroup: 4 waves, each a 32x32 output block of the 64x64 tile.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// One operand tile of the K step starting at k0 in registers: 512 groups of
+// four FP32 values (two per thread), each four consecutive elements along
+// the operand's unit-stride dimension (the rows of the tile, R = 64, or its
+// K = 32 columns), read as one 16-byte load where the group is aligned and
+// inside the matrix.
+struct TileRegs {
+  float v[2][4];
+};
+
+// element (r, k) of an operand with strides (s_r, s_k) and extents (nr, nk);
+// `kfast`: the K direction has unit stride (groups run along k)
+__device__ __forceinline__ void load_tile(TileRegs& t, const float* P, long long s_r, long long s_k, long long r0,
+                                          long long k0, long long nr, long long nk, bool kfast, int tid) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int c = tid + h * 256;  // group 0..511
+    // kfast: group (r = c / 8, k = 4 (c % 8)); else (k = c / 16, r = 4 (c % 16))
+    const int r = kfast ? c >> 3 : (c & 15) * 4;
+    const int k = kfast ? (c & 7) * 4 : c >> 4;
+    const long long gr = r0 + r, gk = k0 + k;
+    const float* q = P + gr * s_r + gk * s_k;
+    const bool whole = kfast ? (gr < nr && gk + 3 < nk) : (gr + 3 < nr && gk < nk);
+    const bool unit = kfast ? s_k == 1 : s_r == 1;
+    if (unit && whole && ((uintptr_t)q & 15) == 0) {
+      const float4 x = *reinterpret_cast<const float4*>(q);
+      t.v[h][0] = x.x;
+      t.v[h][1] = x.y;
+      t.v[h][2] = x.z;
+      t.v[h][3] = x.w;
+    } else {
+      const long long step = kfast ? s_k : s_r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool in = kfast ? (gr < nr && gk + e < nk) : (gr + e < nr && gk < nk);
+        t.v[h][e] = in ? q[e * step] : 0.f;
+      }
+    }
+  }
+}
+
+// the registers of load_tile into the LDS tile [R][KP] as BF16 (round to
+// nearest even): a k-run is one 8-byte store, an r-run four 2-byte stores
+__device__ __forceinline__ void store_tile(const TileRegs& t, unsigned short* S, bool kfast, int tid) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int c = tid + h * 256;
+    const int r = kfast ? c >> 3 : (c & 15) * 4;
+    const int k = kfast ? (c & 7) * 4 : c >> 4;
+    if (kfast) {
+      const uint2 w = make_uint2((uint32_t)to_bf16(t.v[h][0]) | ((uint32_t)to_bf16(t.v[h][1]) << 16),
+                                 (uint32_t)to_bf16(t.v[h][2]) | ((uint32_t)to_bf16(t.v[h][3]) << 16));
+      *reinterpret_cast<uint2*>(S + r * KP + k) = w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) S[(r + e) * KP + k] = to_bf16(t.v[h][e]);
+    }
+  }
+}
+
+// Workgroup: 4 waves, each a 32x32 output block of the 64x64 tile, on
+// v_mfma_f32_32x32x16_bf16 (gfx950: 16 products of the K step per
+// instruction, two instructions per wave and K step).  The next K step's
+// operands are read from HBM into registers while the current step's
+// products run; the LDS tiles are double-buffered, one barrier per step.
 __global__ void __launch_bounds__(256) gemm_kernel(wfno_gemm_args g) {
-  __shared__ unsigned short As[TM * KP];  // [m][k]
-  __shared__ unsigned short Bs[TN * KP];  // [n][k]
+  __shared__ __align__(16) unsigned short As[2][TM * KP];  // [m][k]
+  __shared__ __align__(16) unsigned short Bs[2][TN * KP];  // [n][k]
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
@@ -47,41 +112,41 @@ __global__ void __launch_bounds__(256) gemm_kernel(wfno_gemm_args g) {
   const float* A = g.A + b1 * g.sab1 + b2 * g.sab2 + b3 * g.sab3;
   const float* B = g.B + b1 * g.sbb1 + b2 * g.sbb2 + b3 * g.sbb3;
   float* Cm = g.C + b1 * g.scb1 + b2 * g.scb2 + b3 * g.scb3;
+  // groups of four along k where K has unit stride, else along the rows
+  const bool akf = g.sak == 1 || g.sam != 1;
+  const bool bkf = g.sbk == 1 || g.sbn != 1;
 
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 
-  for (long long k0 = 0; k0 < g.K; k0 += TK) {
-    // stage A[m0:m0+64, k0:k0+32] and B[k0:k0+32, n0:n0+64] as bf16
-    for (int e = tid; e < TM * TK; e += 256) {
-      int mi, ki;
-      if (g.sak == 1) { mi = e / TK; ki = e % TK; } else { ki = e / TM; mi = e % TM; }
-      const long long m = m0 + mi, k = k0 + ki;
-      const float v = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.f;
-      As[mi * KP + ki] = to_bf16(v);
+  TileRegs ra, rb;
+  load_tile(ra, A, g.sam, g.sak, m0, 0, g.M, g.K, akf, tid);
+  load_tile(rb, B, g.sbn, g.sbk, n0, 0, g.N, g.K, bkf, tid);
+  store_tile(ra, As[0], akf, tid);
+  store_tile(rb, Bs[0], bkf, tid);
+  __syncthreads();
+  const int nk = (int)((g.K + TK - 1) / TK);
+  for (int s = 0; s < nk; ++s) {
+    const int cur = s & 1;
+    const bool more = s + 1 < nk;
+    if (more) {
+      load_tile(ra, A, g.sam, g.sak, m0, (long long)(s + 1) * TK, g.M, g.K, akf, tid);
+      load_tile(rb, B, g.sbn, g.sbk, n0, (long long)(s + 1) * TK, g.N, g.K, bkf, tid);
     }
-    for (int e = tid; e < TN * TK; e += 256) {
-      int ni, ki;
-      if (g.sbk == 1) { ni = e / TK; ki = e % TK; } else { ki = e / TN; ni = e % TN; }
-      const long long n = n0 + ni, k = k0 + ki;
-      const float v = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.f;
-      Bs[ni * KP + ki] = to_bf16(v);
+    // lane holds A[i][kk + 8 (lane / 32) + j] and B[kk + 8 (lane / 32) + j][n], j < 8
+    const int i = wm * 32 + (lane & 31);
+    const int n = wn * 32 + (lane & 31);
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 16) {
+      const int kb = kk + 8 * (lane >> 5);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[cur][i * KP + kb]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[cur][n * KP + kb]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 8) {
-      // lane holds A[i][kk + 4*(lane/32) + j] and B[kk + 4*(lane/32) + j][n], j = 0..3
-      const int i = wm * 32 + (lane & 31);
-      const int n = wn * 32 + (lane & 31);
-      const int kb = kk + 4 * (lane >> 5);
-      bf16x4 a, b;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a[j] = (short)As[i * KP + kb + j];
-        b[j] = (short)Bs[n * KP + kb + j];
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    if (more) {
+      store_tile(ra, As[cur ^ 1], akf, tid);
+      store_tile(rb, Bs[cur ^ 1], bkf, tid);
     }
     __syncthreads();
   }
