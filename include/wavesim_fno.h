@@ -64,6 +64,14 @@ int wfno_features(const float* n, const float* phase, float* X, const int roi_n[
  * roi_lo and roi_n in [y x z] order, E0 in the solver layout. */
 int wfno_compose_prior(void* E0, const float* A, const void* P, const void* u, const int grid[3],
                        const int roi_lo[3], const int roi_n[3], int factor, const int foot[4], void* stream);
+/* Real form of the per-mode complex channel mixing: for each of npairs
+ * weight pairs (Wr [modes][width][width] followed by Wi of the same shape,
+ * pair g at W + (g / per_group) * group_stride + (g % per_group) * pair_stride
+ * elements) and each mode k, Wc[g][k] = [[Wr_k, -Wi_k], [Wi_k, Wr_k]]
+ * (2 width x 2 width, row-major), so that [Yr; Yi] = Wc_k [Fr; Fi] is
+ * Y = (Wr + i Wi) F in one wfno_gemm. */
+int wfno_complex_weights(const float* W, float* Wc, long long pair_stride, int per_group, long long group_stride,
+                         int npairs, int modes, int width, void* stream);
 /* dst[i] (+)= sum_{c < nchunks} src[c*n + i]  (fixed order) */
 int wfno_reduce_chunks(const float* src, float* dst, int nchunks, long long n, int accumulate, void* stream);
 const char* wfno_last_error(void);

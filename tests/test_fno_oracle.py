@@ -75,3 +75,23 @@ def test_bf16_rounding_round_to_nearest_even():
     # 1 + 3*2^-9: above half -> 1 + 2^-7; 1 + 3*2^-8: tie -> even (1 + 2^-6)
     assert list(r[:5]) == [1.0, 1.0, 1.0, 1 + 2 ** -7, 1 + 2 ** -6]
     assert r[5] == -3.0
+
+
+def test_network_dft_matrices_interleave_real_and_imaginary_parts():
+    """The GPU network's truncated DFT keeps each mode's real and imaginary
+    part adjacent (columns 2k, 2k+1), so that one GEMM with the 2W x 2W real
+    form of the complex weight mixes a mode; checked against numpy.fft."""
+    from wavesim_amd.learned.fno import dft_matrices
+    L, m = 16, 5
+    D, Dinv = dft_matrices(L, m)
+    assert D.shape == (L, 2 * m) and Dinv.shape == (2 * m, L)
+    rng = np.random.default_rng(2)
+    h = rng.standard_normal(L)
+    F = h @ D
+    assert np.allclose(F[0::2] + 1j * F[1::2], np.fft.rfft(h)[:m])
+    Y = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    Y[0] = Y[0].real
+    full = np.zeros(L // 2 + 1, complex)
+    full[:m] = Y
+    inter = np.stack([Y.real, Y.imag], axis=1).ravel()
+    assert np.allclose(inter @ Dinv, np.fft.irfft(full, L))

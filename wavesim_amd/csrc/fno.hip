@@ -163,6 +163,26 @@ __global__ void __launch_bounds__(256) gemm_kernel(wfno_gemm_args g) {
   }
 }
 
+// Wc[g][k] = [[Wr_k, -Wi_k], [Wi_k, Wr_k]] (2w x 2w) for every mode k of
+// pair g: the complex mixing Y_k = (Wr_k + i Wi_k) F_k as one real product
+// [Yr; Yi] = Wc_k [Fr; Fi].  Pair g's Wr [m][w][w] starts at
+// W + (g / per_group) group_stride + (g % per_group) pair_stride, its Wi
+// right after it.
+__global__ void complex_weights_kernel(const float* W, float* Wc, long long pair_stride, int per_group,
+                                       long long group_stride, long long total, int m, int w) {
+  const long long blk = 4LL * w * w, per = (long long)m * blk;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long g = i / per, k = (i % per) / blk;
+    const int rc = (int)(i % blk), r = rc / (2 * w), c = rc % (2 * w);
+    const float* Wr = W + (g / per_group) * group_stride + (g % per_group) * pair_stride;
+    const bool rh = r >= w, ch = c >= w;
+    const long long e = k * w * w + (long long)(r - (rh ? w : 0)) * w + (c - (ch ? w : 0));
+    const float v = rh == ch ? Wr[e] : Wr[(long long)m * w * w + e];
+    Wc[i] = (!rh && ch) ? -v : v;
+  }
+}
+
 __global__ void reduce_chunks_kernel(const float* src, float* dst, int nchunks, long long n, int accumulate) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -406,6 +426,20 @@ int wfno_reduce_chunks(const float* src, float* dst, int nchunks, long long n, i
   return guard([&]() -> int {
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, src, dst,
                        nchunks, n, accumulate);
+    return 0;
+  });
+}
+
+int wfno_complex_weights(const float* W, float* Wc, long long pair_stride, int per_group, long long group_stride,
+                         int npairs, int modes, int width, void* stream) {
+  return guard([&]() -> int {
+    if (npairs <= 0 || per_group <= 0 || modes <= 0 || width <= 0) {
+      g_err = "bad complex weight shape";
+      return -1;
+    }
+    const long long n = (long long)npairs * modes * 4 * width * width;
+    hipLaunchKernelGGL(complex_weights_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, W, Wc,
+                       pair_stride, per_group, group_stride, n, modes, width);
     return 0;
   });
 }

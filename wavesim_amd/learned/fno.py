@@ -5,14 +5,18 @@ Architecture (cubic grid L^3, activations [B][C][Z][Y][X], x fastest):
   layer l   h = GELU( sum_a K_a,l h + Wp_l h + b_l ),  a in {x, y, z}
   project   y = Q2 GELU(Q1 h + c1) + c2            (width -> proj -> 2)
 K_a: first m Fourier modes along axis a (truncated DFT as a GEMM with the
-[L][2m] cos/-sin matrix D), per-mode complex channel mixing (Wr, Wi of shape
-[m][W][W]), Hermitian inverse DFT over those modes (GEMM with Dinv [2m][L]).
+[L][2m] matrix D whose columns 2k, 2k+1 are cos and -sin of mode k),
+per-mode complex channel mixing (Wr, Wi of shape [m][W][W]; one real GEMM per
+mode with the 2W x 2W weight [[Wr, -Wi], [Wi, Wr]] on the mode's real and
+imaginary parts, which the interleaved columns of D keep adjacent), Hermitian
+inverse DFT over those modes (GEMM with Dinv [2m][L], rows interleaved alike).
 This is the factorised ("F-FNO") spectral layer: 3 * 2 * m * W^2 weights per
 layer instead of 2 * m^3 * W^2 for a dense 3-D mode block.
 
 Every matrix product (forward and backward) is one launch of wfno_gemm (BF16
-MFMA, FP32 accumulate); bias, GELU, loss and Adam are the project's
-elementwise kernels.  torch tensors only own device memory.
+MFMA, FP32 accumulate); bias, GELU, loss, Adam and the assembly of the
+2W x 2W mixing weights are the project's elementwise kernels.  torch tensors
+only own device memory.
 """
 from __future__ import annotations
 
@@ -36,12 +40,15 @@ class Config:
 
 
 def dft_matrices(L: int, m: int):
+    """D [L][2m]: column 2k = cos, 2k+1 = -sin of mode k (real and imaginary
+    part of the truncated DFT); Dinv [2m][L]: the Hermitian inverse over
+    those modes, rows in the same interleaved order."""
     x = np.arange(L)[:, None]
     k = np.arange(m)[None, :]
     ang = 2 * np.pi * x * k / L
-    D = np.concatenate([np.cos(ang), -np.sin(ang)], axis=1)
+    D = np.stack([np.cos(ang), -np.sin(ang)], axis=2).reshape(L, 2 * m)
     ck = np.where(np.arange(m) == 0, 1.0, 2.0)[:, None] / L
-    Dinv = np.concatenate([ck * np.cos(ang.T), -ck * np.sin(ang.T)], axis=0)
+    Dinv = np.stack([ck * np.cos(ang.T), -ck * np.sin(ang.T)], axis=1).reshape(2 * m, L)
     return D, Dinv
 
 
@@ -156,7 +163,8 @@ class FNO:
         K.check(K.lib().wfno_gelu_bwd(K._addr(dout), K._addr(pre), K._addr(dpre), n, K.stream_ptr()))
 
     # transforms along an axis: "fwd" maps h-layout [B][C][L^3] to the mode
-    # layout [B][2m][C][L^2]; "inv" maps back (accumulating)
+    # layout [B][m][2][C][L^2] (real, imaginary part of each mode adjacent);
+    # "inv" maps back (accumulating)
     def to_modes(self, a, Mat, s_rk, s_x, src, dst, C, B):
         L, m = self.cfg.L, self.cfg.modes
         V, L2, r = L ** 3, L * L, 2 * self.cfg.modes
@@ -182,34 +190,40 @@ class FNO:
             K.gemm(L, L2, r, Mat, (s_x, s_rk, 0, 0), src, (C * L2, 1, r * C * L2, L2),
                    dst, (L2, 1, C * V, V), beta=beta, nb=(B, C))
 
-    def mix(self, Wr, Wi, F, Y, B, transpose=False):
-        """Per-mode complex channel mixing in the mode layout [B][2m][W][L^2]:
-        Y = (Wr + i Wi) F, or with transpose=True Y = (Wr + i Wi)^H-style
-        adjoint  dF = Wr^T dY_r + Wi^T dY_i, Wr^T dY_i - Wi^T dY_r."""
+    def complex_weights(self):
+        """Wc [layers][3][m][2W][2W] = [[Wr, -Wi], [Wi, Wr]] per layer, axis
+        and mode, from the current parameters (one launch)."""
+        cfg = self.cfg
+        W, m = cfg.width, cfg.modes
+        Wc = self.buf("Wc", cfg.layers * 3 * m * 4 * W * W)
+        o0 = self.offsets["Wr0x"][0]
+        pair = self.offsets["Wr0y"][0] - o0
+        group = self.offsets["Wr1x"][0] - o0 if cfg.layers > 1 else 3 * pair
+        for l in range(cfg.layers):
+            for j, a in enumerate("xyz"):
+                assert self.offsets[f"Wr{l}{a}"][0] == o0 + l * group + j * pair
+                assert self.offsets[f"Wi{l}{a}"][0] == o0 + l * group + j * pair + m * W * W
+        K.check(K.lib().wfno_complex_weights(K._addr((self.flat, o0)), K.ptr(Wc), pair, 3, group,
+                                             3 * cfg.layers, m, W, K.stream_ptr()))
+        return Wc
+
+    def mix(self, Wc, F, Y, B, transpose=False):
+        """Per-mode complex channel mixing in the mode layout [B][m][2][W][L^2]:
+        [Yr; Yi]_k = Wc_k [Fr; Fi]_k with Wc_k = [[Wr, -Wi], [Wi, Wr]], i.e.
+        Y = (Wr + i Wi) F, one GEMM (2W x L^2 x 2W per mode); transpose=True
+        applies Wc_k^T, the adjoint dF = Wr^T dY_r + Wi^T dY_i,
+        Wr^T dY_i - Wi^T dY_r."""
         W, m, L2 = self.cfg.width, self.cfg.modes, self.cfg.L ** 2
-        sW = (1, W, 0, W * W) if transpose else (W, 1, 0, W * W)
-        sF = (L2, 1, 2 * m * W * L2, W * L2)
-        im = m * W * L2
-        Fr, Fi = (F, 0), (F, im)
-        Yr, Yi = (Y, 0), (Y, im)
-        g = lambda A, Bm, Cm, alpha, beta: K.gemm(W, L2, W, A, sW, Bm, sF, Cm, sF, alpha, beta, nb=(B, m))
-        if not transpose:
-            g(Wr, Fr, Yr, 1.0, 0.0)
-            g(Wi, Fi, Yr, -1.0, 1.0)
-            g(Wi, Fr, Yi, 1.0, 0.0)
-            g(Wr, Fi, Yi, 1.0, 1.0)
-        else:
-            g(Wr, Fr, Yr, 1.0, 0.0)     # here F = dY, Y = dF
-            g(Wi, Fi, Yr, 1.0, 1.0)
-            g(Wr, Fi, Yi, 1.0, 0.0)
-            g(Wi, Fr, Yi, -1.0, 1.0)
+        sW = (1, 2 * W, 0, 4 * W * W) if transpose else (2 * W, 1, 0, 4 * W * W)
+        sF = (L2, 1, 2 * m * W * L2, 2 * W * L2)
+        K.gemm(2 * W, L2, 2 * W, Wc, sW, F, sF, Y, sF, nb=(B, m))
 
     def mix_dW(self, dY, F, dWr, dWi, B):
         """dWr = sum dYr Fr^T + dYi Fi^T,  dWi = sum dYi Fr^T - dYr Fi^T."""
         W, m, L2 = self.cfg.width, self.cfg.modes, self.cfg.L ** 2
-        im = m * W * L2
-        sA = (L2, 1, 0, W * L2)           # dY_k[b] as (o, line)
-        sB = (1, L2, 0, W * L2)           # F_k[b]^T as (line, c)
+        im = W * L2                       # imaginary part of a mode right after its real part
+        sA = (L2, 1, 0, 2 * W * L2)       # dY_k[b] as (o, line)
+        sB = (1, L2, 0, 2 * W * L2)       # F_k[b]^T as (line, c)
         sC = (W, 1, 0, W * W)
         for b in range(B):
             o = b * 2 * m * W * L2
@@ -233,14 +247,16 @@ class FNO:
         self.pointwise(self.P("Wl"), x, h, W, cfg.cin, B)
         self.bias_act(h, h, self.P("bl"), W, B, act=False)
         Y = self.buf("Y", B * r * W * L2)
+        Wc = self.complex_weights()
+        nc = 4 * W * W * cfg.modes
         for l in range(cfg.layers):
             cache[f"h{l}"] = h
             pre = torch.empty(B * W * V, device=self.dev)
             self.pointwise(self.P(f"Wp{l}"), h, pre, W, W, B)
-            for a in "xyz":
+            for j, a in enumerate("xyz"):
                 F = torch.empty(B * r * W * L2, device=self.dev) if keep else self.buf("F", B * r * W * L2)
                 self.to_modes(a, D, *s_D, h, F, W, B)
-                self.mix(self.P(f"Wr{l}{a}"), self.P(f"Wi{l}{a}"), F, Y, B)
+                self.mix((Wc, (3 * l + j) * nc), F, Y, B)
                 self.from_modes(a, Dinv, *s_Di, Y, pre, W, B, beta=1.0)
                 if keep:
                     cache[f"F{l}{a}"] = F
@@ -279,6 +295,8 @@ class FNO:
         dpre = self.buf("dpre", B * W * V)
         dY = self.buf("dY", B * r * W * L2)
         dF = self.buf("dF", B * r * W * L2)
+        Wc = self.complex_weights()
+        nc = 4 * W * W * cfg.modes
         for l in reversed(range(cfg.layers)):
             self.gelu_bwd(dh, cache[f"pre{l}"], dpre, B * W * V)
             self.bias_grad(dpre, self.G(f"b{l}"), W, B)
@@ -286,11 +304,11 @@ class FNO:
             self.pointwise_dW(dpre, h, self.G(f"Wp{l}"), W, W, B)
             # dh = Wp^T dpre + sum_a spectral adjoints
             K.gemm(W, V, W, self.P(f"Wp{l}"), (1, W, 0), dpre, (V, 1, W * V), dh, (V, 1, W * V), nb=(B,))
-            for a in "xyz":
+            for j, a in enumerate("xyz"):
                 F = cache[f"F{l}{a}"]
                 self.to_modes(a, self.Dinv, *s_DT, dpre, dY, W, B)
                 self.mix_dW(dY, F, self.G(f"Wr{l}{a}"), self.G(f"Wi{l}{a}"), B)
-                self.mix(self.P(f"Wr{l}{a}"), self.P(f"Wi{l}{a}"), dY, dF, B, transpose=True)
+                self.mix((Wc, (3 * l + j) * nc), dY, dF, B, transpose=True)
                 self.from_modes(a, self.D, *s_D, dF, dh, W, B, beta=1.0)
         self.pointwise_dW(dh, cache["x"], self.G("Wl"), W, cfg.cin, B)
         self.bias_grad(dh, self.G("bl"), W, B)

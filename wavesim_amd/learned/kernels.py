@@ -34,6 +34,7 @@ SYMBOLS = {
     "wfno_bias_grad": [P, P, LL, LL, LL, C.c_int, P],
     "wfno_mse": [P, P, P, P, C.c_int, LL, C.c_float, P],
     "wfno_reduce_chunks": [P, P, C.c_int, LL, C.c_int, P],
+    "wfno_complex_weights": [P, P, LL, C.c_int, LL, C.c_int, C.c_int, C.c_int, P],
     "wfno_wkb_prior": [P, P, P, P, C.POINTER(C.c_int), C.c_float, C.c_float, P],
     "wfno_features": [P, P, P, C.POINTER(C.c_int), C.c_int, C.c_float, P],
     "wfno_compose_prior": [P, P, P, P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
