@@ -289,12 +289,88 @@ __global__ void __launch_bounds__(kControlThreads) control_kernel(DevState* st, 
 // cycle: the stop test applies at multiples of Nw only (Simulation.m:202), so
 // no rhs stops inside a cycle except on the failure exit (whose iteration is
 // still the one recorded)
+//
+// A single rhs with at most one partial per thread and at most kCycleFast
+// iterations takes a short path: all L energies are reduced at once (the same
+// order of additions as control_one, so the energies are bit-identical), and
+// thread 0 then runs the L stop decisions with the state in registers, so the
+// steps do not wait on each other's global-memory round trips.
+constexpr int kCycleFast = 8;
 __global__ void __launch_bounds__(kControlThreads) control_cycle_kernel(DevState* st, const double* partials, int nblk,
                                                                         int L, long long slot) {
-  for (int j = 0; j < L; ++j) {
-    control_one(st, partials + (size_t)j * slot, nblk);
-    __syncthreads();
+  if (st->nrhs != 1 || nblk > kControlThreads || L > kCycleFast) {
+    for (int j = 0; j < L; ++j) {
+      control_one(st, partials + (size_t)j * slot, nblk);
+      __syncthreads();
+    }
+    return;
   }
+  constexpr int NW = kControlThreads / 64;
+  __shared__ double red[kCycleFast][NW];
+  double v[kCycleFast];
+#pragma unroll
+  for (int j = 0; j < kCycleFast; ++j) {
+    v[j] = 0.0;
+    if (j < L && (int)threadIdx.x < nblk) v[j] += partials[(size_t)j * slot + threadIdx.x];
+  }
+#pragma unroll
+  for (int j = 0; j < kCycleFast; ++j) {
+    if (j < L) {
+      double e = v[j];
+      for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+      if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = e;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0 || st->all_stopped) return;
+  // the decisions of control_one for rhs 0, state in registers
+  int it = st->it, calc = st->calc_energy, stopped = st->stopped[0], all_stopped = 0;
+  double last = st->last_energy[0], diff0 = st->diff[0];
+  const double ref = st->ref_energy[0], thr = st->threshold;
+  const int nw = st->nw, interval = st->interval, hist_cap = st->hist_cap;
+  const long long max_iter = st->max_iter;
+  double* ea = st->energy_all;
+  for (int j = 0; j < L && !all_stopped; ++j) {
+    int all = 1;
+    if (!stopped) {
+      double en = 0.0;
+      for (int k = 0; k < NW; ++k) en += red[j][k];
+      ea[it - 1] = en;
+      if (calc) last = en;
+      const double de = last;
+      st->diff[it - 1] = de;
+      if (it == 1) diff0 = de;
+      const bool can = (it % nw) == 0;
+      const double d1 = ref > 0.0 ? ref : diff0;
+      bool diverged = !isfinite(en);
+      if (can && it > 2 * nw) {
+        const double e1 = ea[it - 1 - nw], e2 = ea[it - 1 - 2 * nw];
+        diverged = diverged || (en > e1 && e1 > e2 && en > 100.0 * fmax(ea[0], ref));
+      }
+      int conv = 2;  // 2: running
+      if (diverged)
+        conv = -1;
+      else if (can && de / d1 < thr)
+        conv = 1;
+      else if ((can && it >= max_iter) || it >= hist_cap)
+        conv = 0;
+      if (conv != 2) {
+        stopped = 1;
+        st->stopped[0] = 1;
+        st->converged[0] = conv;
+        st->n_iter[0] = it;
+      }
+      all = stopped;
+    }
+    calc = ((it - 1) % interval) == 0;
+    ++it;
+    all_stopped = all;
+  }
+  st->last_energy[0] = last;
+  st->calc_energy = calc;
+  st->it = it;
+  st->all_stopped = all_stopped;
+  if (st->host_stopped != nullptr && all_stopped) *st->host_stopped = 1;
 }
 
 // grouped launches: workgroup e runs the control step of problem e and
