@@ -212,6 +212,25 @@ def learned_ab(media: int):
     return out
 
 
+def learned_forward(count: int):
+    """`count` eager forwards of the learned warm start's network (batch 1, the
+    shipped weights, zero input) after one that allocates its buffers; wall
+    time of each from a synchronised start."""
+    import torch
+    from wavesim_amd.learned.warmstart import DEFAULT_WEIGHTS, WarmStart, CIN
+    ws = WarmStart.load(DEFAULT_WEIGHTS)
+    X = torch.zeros(1, CIN, ws.cfg.L ** 3, dtype=torch.float32, device=ws.dev)
+    ws.net.forward(X, keep=False)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(count):
+        t0 = time.perf_counter()
+        ws.net.forward(X, keep=False)
+        torch.cuda.synchronize()
+        ms.append(round(1e3 * (time.perf_counter() - t0), 3))
+    return {"mode": "learned_forward", "parameters": ws.net.nparam, "grid": ws.cfg.L, "forward_ms": ms}
+
+
 def dropin(sim, src, steps: int, workload: str):
     """The MEX-shaped entry point wsim_run_algorithm end to end: host gamma,
     wiggle tables and source in, host E' and diff_energy out, a device solver
@@ -357,7 +376,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="keep the default FFT plan when none is cached")
     ap.add_argument("--plan", default="", help="FFT plan text to use instead of the cached/tuned one")
-    ap.add_argument("--learned-media", type=int, default=4,
+    ap.add_argument("--learned-media", type=int, default=8,
                     help="held-out media of the learned warm-start A/B (0: skip)")
     ap.add_argument("--trace-iterations", type=int, default=0,
                     help="run this many iterations of the workload and exit (counter collection)")
@@ -370,6 +389,9 @@ def main():
     ap.add_argument("--pstd", type=int, default=0,
                     help="time this many PSTD steps (PSTD.m) on the workload's medium with absorption "
                          "Im n = 0.02 and exit")
+    ap.add_argument("--learned-forward", type=int, default=0,
+                    help="run this many forwards of the learned warm start's network (eager) and exit "
+                         "(kernel traces of the network)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the 1 GiB copy whose counter readings calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -390,6 +412,9 @@ def main():
     from wavesim_amd.workloads import WORKLOADS
     if args.pstd > 0:
         print(json.dumps(pstd_rate(args.workload, args.pstd, args.steps)), flush=True)
+        return
+    if args.learned_forward > 0:
+        print(json.dumps(learned_forward(args.learned_forward)), flush=True)
         return
     if args.f64:
         print(json.dumps(f64_rate(args.workload, args.steps, args.profile_reps)), flush=True)

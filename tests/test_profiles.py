@@ -54,3 +54,33 @@ def test_profiles_measured_on_the_shipped_plan():
     assert _bench().load_traffic("disordered_3d_256", plan)
     head = open(os.path.join(ROOT, "profiles", "kernel_stats_disordered_3d_256.txt")).read(2000)
     assert plan in head
+
+
+def test_network_gemm_roles_follow_the_forward(monkeypatch):
+    """The role list the trace summary labels a forward's GEMM launches with
+    matches the launches FNO.forward makes (run on the CPU with the kernel
+    library stubbed out, recording each GEMM's shape)."""
+    import torch
+    from wavesim_amd.learned import fno, kernels as K
+
+    class Stub:
+        def __getattr__(self, name):
+            return lambda *a: 0
+
+    calls = []
+    monkeypatch.setattr(K, "lib", lambda: Stub())
+    monkeypatch.setattr(K, "stream_ptr", lambda stream=None: None)
+    monkeypatch.setattr(K, "gemm", lambda M, N, Kd, *a, **k: calls.append((M, N, Kd)))
+    L, W, m = 8, 4, 2
+    cfg = fno.Config(L=L, cin=6, width=W, modes=m, layers=2, proj=8)
+    net = fno.FNO(cfg, device="cpu")
+    net.forward(torch.zeros(1, 6, L ** 3), keep=False)
+    roles = fno.forward_gemm_roles(cfg)
+    assert len(calls) == len(roles) == 3 + 2 * 10
+    shapes = {"lift": (W, L ** 3, 6), "pointwise Wp": (W, L ** 3, W), "complex mixing": (2 * W, L * L, 2 * W),
+              "truncated DFT x": (2 * m, W * L * L, L), "inverse truncated DFT x": (W * L * L, L, 2 * m),
+              "truncated DFT y": (2 * m, L, L), "inverse truncated DFT y": (L, L, 2 * m),
+              "truncated DFT z": (2 * m, L * L, L), "inverse truncated DFT z": (L, L * L, 2 * m),
+              "projection Q1": (8, L ** 3, W), "projection Q2": (2, L ** 3, 8)}
+    for role, shape in zip(roles, calls):
+        assert shapes[role] == shape, role

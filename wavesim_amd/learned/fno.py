@@ -77,6 +77,17 @@ def init_params(cfg: Config, seed: int = 0) -> dict:
     return p
 
 
+def forward_gemm_roles(cfg: Config):
+    """What each wfno_gemm launch of FNO.forward computes, in launch order
+    (for reading kernel traces of a forward)."""
+    roles = ["lift"]
+    for _ in range(cfg.layers):
+        roles.append("pointwise Wp")
+        for a in "xyz":
+            roles += [f"truncated DFT {a}", "complex mixing", f"inverse truncated DFT {a}"]
+    return roles + ["projection Q1", "projection Q2"]
+
+
 class FNO:
     """Parameters live in one flat device buffer (one Adam launch)."""
 
@@ -243,12 +254,12 @@ class FNO:
         s_D = (1, r)        # D[x][rk] as Mat(rk, x)
         s_Di = (cfg.L, 1)   # Dinv[rk][x]
         cache = {"x": x, "B": B}
+        Wc = self.complex_weights()
+        nc = 4 * W * W * cfg.modes
         h = torch.empty(B * W * V, device=self.dev)
         self.pointwise(self.P("Wl"), x, h, W, cfg.cin, B)
         self.bias_act(h, h, self.P("bl"), W, B, act=False)
         Y = self.buf("Y", B * r * W * L2)
-        Wc = self.complex_weights()
-        nc = 4 * W * W * cfg.modes
         for l in range(cfg.layers):
             cache[f"h{l}"] = h
             pre = torch.empty(B * W * V, device=self.dev)

@@ -118,6 +118,38 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str) -> str:
     return "\n".join(lines)
 
 
+def fno_forward(d: str, layers: int = 4) -> str:
+    """Per-role GEMM times of the last network forward in a kernel trace (a
+    forward starts with its complex_weights launch; its GEMMs follow in the
+    order of wavesim_amd.learned.fno.forward_gemm_roles)."""
+    from .learned.fno import Config, forward_gemm_roles
+    with open(_one("*kernel_trace.csv", d)) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if "complex_weights_kernel" in r["Kernel_Name"]]
+    if not starts:
+        raise ValueError(f"no network forward in the trace under {d}")
+    roles = forward_gemm_roles(Config(layers=layers))
+    fwd = rows[starts[-1]:]
+    gemms = [r for r in fwd if "gemm_kernel" in r["Kernel_Name"]][:len(roles)]
+    if len(gemms) != len(roles):
+        raise ValueError(f"{len(gemms)} GEMM launches after the last forward's start, {len(roles)} expected")
+    dur = lambda r: (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    other = defaultdict(list)
+    for r in fwd:
+        if "gemm_kernel" not in r["Kernel_Name"]:
+            other[r["Kernel_Name"].split("(")[0].split("::")[-1]].append(dur(r))
+    acc = defaultdict(list)
+    for role, r in zip(roles, gemms):
+        acc[role].append(dur(r))
+    total = sum(dur(r) for r in gemms)
+    lines = [f"# last forward: {len(gemms)} GEMM launches {total:.1f} us, other kernels "
+             + ", ".join(f"{k} {len(v)}x {sum(v):.1f} us" for k, v in other.items()),
+             f"{'role':28s} {'launches':>8s} {'avg_us':>8s} {'total_us':>9s}"]
+    for role, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{role:28s} {len(v):8d} {sum(v) / len(v):8.2f} {sum(v):9.1f}")
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -126,9 +158,14 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     for name in ("fetch", "write", "copy-fetch", "copy-write"):
         t.add_argument("--" + name, required=True)
+    n = sub.add_parser("fno")
+    n.add_argument("dir")
+    n.add_argument("--layers", type=int, default=4)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats(a.dir))
+    elif a.cmd == "fno":
+        print(fno_forward(a.dir, a.layers))
     else:
         print(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write))
     return 0
