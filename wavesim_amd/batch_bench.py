@@ -2,6 +2,7 @@
 
     python -m wavesim_amd.batch_bench ragged [--count 32]   # independent 2-D problems, ragged grids
     python -m wavesim_amd.batch_bench rhs [--rhs 64 --n 192]  # several sources on one 3-D medium
+    python -m wavesim_amd.batch_bench rhs --trace-iterations 24  # the batched solver only (counters)
 
 Solves return once their results are final (no extra synchronisation is
 needed around the clocks).  Each mode times the solves only (media, sources and plans set up before the
@@ -61,7 +62,7 @@ def ragged(count: int, boundary: int, reps: int):
             "voxel_updates_per_s_sequential": round(upd / min(single), 1)}
 
 
-def multi_rhs(nrhs: int, n: int, boundary: int, reps: int):
+def multi_rhs(nrhs: int, n: int, boundary: int, reps: int, trace_iterations: int = 0):
     from . import WaveSim
     from .simulation import Source
     from .workloads import disordered_3d
@@ -71,10 +72,15 @@ def multi_rhs(nrhs: int, n: int, boundary: int, reps: int):
     sim = WaveSim(nmap, opt)
     mi = sim.max_iterations()
     prepared = [sim._prepare(s) for s in srcs]
-    one = WaveSim(nmap, opt).solver(1)
     batch = sim.solver(nrhs)
     for r, p in enumerate(prepared):
         batch.set_source(r, p.positions, p.values)
+    if trace_iterations > 0:
+        # only the batched solver, a fixed number of iterations (counter collection)
+        batch.solve(trace_iterations, 0.0)
+        return {"mode": "rhs-trace", "rhs": nrhs, "grid": [int(x) for x in sim.N],
+                "iterations": batch.result(0).iterations, "plan": batch.plan}
+    one = WaveSim(nmap, opt).solver(1)
     single, its_single = [], []
     for p in prepared[:1]:  # warm-up
         one.set_source(0, p.positions, p.values)
@@ -110,11 +116,14 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--boundary", type=int, default=32)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--trace-iterations", type=int, default=0,
+                    help="rhs mode: run only the batched solver for this many iterations and exit "
+                         "(counter collection)")
     a = ap.parse_args(argv)
     if a.mode == "ragged":
         out = ragged(a.count, a.boundary, a.reps)
     else:
-        out = multi_rhs(a.rhs, a.n, a.boundary, a.reps)
+        out = multi_rhs(a.rhs, a.n, a.boundary, a.reps, a.trace_iterations)
     print(json.dumps(out), flush=True)
 
 

@@ -71,13 +71,18 @@ def kernel_durations(d: str):
     return out
 
 
-def counter_means(d: str, counter: str):
-    """{kernel: mean counter value per dispatch} from *_counter_collection.csv."""
+def counter_means(d: str, counter: str, skip_idle: bool = False):
+    """{kernel: (mean counter value per dispatch, dispatches)} from
+    *_counter_collection.csv.  skip_idle: leave out dispatches reading below 1%
+    of the kernel's largest (the launches queued after a solve stopped, which
+    return at once)."""
     acc = defaultdict(list)
     with open(_one("*counter_collection.csv", d)) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] == counter:
                 acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    if skip_idle:
+        acc = {k: [x for x in v if x >= 0.01 * max(v)] for k, v in acc.items()}
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
@@ -105,7 +110,7 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str) -> str:
     sf, sw = COPY_BYTES / cf, COPY_BYTES / cw  # bytes per KiB reading
     lines = [f"# copy calibration: 1 GiB copy reads FETCH_SIZE {cf:.1f} KiB -> {sf:.1f} B per unit, "
              f"WRITE_SIZE {cw:.1f} KiB -> {sw:.1f} B per unit"]
-    fm, wm = counter_means(fetch, "FETCH_SIZE"), counter_means(write, "WRITE_SIZE")
+    fm, wm = counter_means(fetch, "FETCH_SIZE", True), counter_means(write, "WRITE_SIZE", True)
     rows = {}
     for k in fm:
         if k in wm and "axis_pass_kernel" in k:
