@@ -158,8 +158,9 @@ def test_shipped_plans_become_a_compiled_table(tmp_path):
     rows = [l for l in build.shipped_plans_inc().splitlines() if l.startswith("{")]
     assert len(rows) == len(table) > 0
     for k, v in table.items():
-        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", k)
-        assert "{" + ", ".join(m.groups()) + f', "{v["plan"]}"}},' in rows
+        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)(\|f64)?", k)
+        f64 = 1 if m.group(7) else 0
+        assert "{" + ", ".join(m.groups()[:6]) + f', {f64}, "{v["plan"]}"}},' in rows
     bad = tmp_path / "bad.json"
     bad.write_text(json.dumps({"320x320|rhs1|nw4": {"plan": "y:5,8,8/2"}}))
     with pytest.raises(ValueError):

@@ -52,3 +52,32 @@ def test_shipped_shapes_are_distinct_and_generate_units():
     assert "FixedPlan<" in src and "wsim_fixed_0" in src
     table = fp.table_unit(shapes)
     assert table.count("wsim_fixed_") == 2 * len(shapes)
+
+
+def test_complex128_plans_of_their_own(tmp_path):
+    """A "|f64" entry gives complex128 solves of that shape their own plan
+    (and plan-specialised kernels of that plan); a shape without one runs its
+    complex64 plan in complex128 too."""
+    import json
+    t = {"320x320x320x1|rhs1|nw8": {"plan": "y:4,16,5/32/512;x:16,20/16/512;z:20,16/16/320"},
+         "320x320x320x1|rhs1|nw8|f64": {"plan": "y:8,8,5/0;x:16,20/16/512;z:4,16,5/8/256"},
+         "192x192x192x1|rhs1|nw8": {"plan": "y:3,16,4/16/256;x:4,16,3/32/256;z:3,8,8/16/0"}}
+    p = tmp_path / "plans.json"
+    p.write_text(json.dumps(t))
+    shapes = fp.shipped_shapes(str(p))
+    mid = lambda f64: {(s[6], s[11]) for s in shapes if s[:2] == (0, 2) and (s[5] == 2) == f64}
+    assert mid(False) == {(320, (20, 16)), (192, (3, 8, 8))}   # the z (middle) passes
+    assert mid(True) == {(320, (4, 16, 5)), (192, (3, 8, 8))}
+
+
+def test_plan_lookup_by_precision(tmp_path, monkeypatch):
+    import json
+    from wavesim_amd import plans
+    cache = tmp_path / "cache.json"
+    cache.write_text(json.dumps({"8x8x8x1|rhs1|nw8": {"plan": "a"}, "8x8x8x1|rhs1|nw8|f64": {"plan": "b"},
+                                 "16x16x16x1|rhs1|nw8": {"plan": "c"}}))
+    monkeypatch.setenv("WSIM_PLAN_CACHE", str(cache))
+    assert plans.key([8, 8, 8, 1], 1, 8, f64=True) == "8x8x8x1|rhs1|nw8|f64"
+    assert plans.lookup([8, 8, 8, 1], 1, 8) == "a"
+    assert plans.lookup([8, 8, 8, 1], 1, 8, f64=True) == "b"
+    assert plans.lookup([16, 16, 16, 1], 1, 8, f64=True) == "c"  # no complex128 plan: the complex64 one

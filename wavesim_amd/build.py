@@ -62,17 +62,18 @@ def _run(cmd):
 
 
 def shipped_plans_inc(path: str = os.path.join(HERE, "tuned_plans.json")) -> str:
-    """C++ initialiser rows {Ny, Nx, Nz, C, nrhs, Nw, "plan"} of the plans in
-    ``path`` (keys as ``plans.key``: "NyxNxxNzxC|rhsR|nwW")."""
+    """C++ initialiser rows {Ny, Nx, Nz, C, nrhs, Nw, f64, "plan"} of the plans
+    in ``path`` (keys as ``plans.key``: "NyxNxxNzxC|rhsR|nwW", "|f64" appended
+    for complex128 solves)."""
     rows = ["// generated by wavesim_amd/build.py from tuned_plans.json"]
     with open(path) as f:
         table = json.load(f)
     for k in sorted(table):
-        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", k)
+        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)(\|f64)?", k)
         plan = table[k]["plan"]
         if not m or not re.fullmatch(r"[xyz0-9:,/;a-z]+", plan):
             raise ValueError(f"bad tuned plan entry {k!r}: {plan!r}")
-        rows.append("{" + ", ".join(m.groups()) + f', "{plan}"}},')
+        rows.append("{" + ", ".join(m.groups()[:6]) + f", {1 if m.group(7) else 0}" + f', "{plan}"}},')
     return "\n".join(rows) + "\n"
 
 

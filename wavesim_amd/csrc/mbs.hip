@@ -189,14 +189,14 @@ constexpr long long kMaxVoxelsF64 = (1ll << 28);
 // that shape is created (the rows are generated from tuned_plans.json by the
 // build; a build without them uses the default plans)
 struct ShippedPlan {
-  int n0, n1, n2, c, nrhs, nw;
+  int n0, n1, n2, c, nrhs, nw, f64;  // f64: the plan of complex128 solvers of that shape
   const char* plan;
 };
 static const ShippedPlan kShippedPlans[] = {
 #if __has_include("shipped_plans.inc")
 #include "shipped_plans.inc"
 #endif
-    {0, 0, 0, 0, 0, 0, nullptr}};
+    {0, 0, 0, 0, 0, 0, 0, nullptr}};
 // Decisions every thread takes (return, skip a stopped rhs) come from one
 // thread's read of the state, broadcast through LDS: the state is written by
 // thread 0 of this workgroup, possibly in the previous step of the same launch
@@ -1487,15 +1487,23 @@ struct Solver {
     HIPCHK(hipMemset(p_stopped, 0, sizeof(int)));
     dalloc(&p_dummy, 1);
     rebuild_launches();
-    for (const ShippedPlan* sp = kShippedPlans; sp->plan; ++sp)
-      if (sp->n0 == N[0] && sp->n1 == N[1] && sp->n2 == N[2] && sp->c == C && sp->nrhs == nrhs && sp->nw == nw) {
-        try {
-          set_plan(sp->plan);
-        } catch (const Error&) {
-          reset_plan();  // keep the default plan
+    // the shipped plan of this shape and precision; a complex128 solver
+    // without one takes the complex64 plan
+    const ShippedPlan* shipped = nullptr;
+    for (int want = f64 ? 1 : 0; want >= 0 && !shipped; --want)
+      for (const ShippedPlan* sp = kShippedPlans; sp->plan; ++sp)
+        if (sp->n0 == N[0] && sp->n1 == N[1] && sp->n2 == N[2] && sp->c == C && sp->nrhs == nrhs && sp->nw == nw &&
+            sp->f64 == want) {
+          shipped = sp;
+          break;
         }
-        break;
+    if (shipped) {
+      try {
+        set_plan(shipped->plan);
+      } catch (const Error&) {
+        reset_plan();  // keep the default plan
       }
+    }
     // the probe state: one rhs, never stopped
     DevState ps{};
     ps.it = nw + 1;

@@ -181,18 +181,23 @@ def shipped_shapes(path: str = os.path.join(HERE, "tuned_plans.json"), extra: st
     radix 20)."""
     with open(path) as f:
         table = json.load(f)
-    items = []
+    items = {False: [], True: []}  # complex64 / complex128 plans
     for key in sorted(table):
-        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)", key)
+        m = re.fullmatch(r"(\d+)x(\d+)x(\d+)x(\d+)\|rhs(\d+)\|nw(\d+)(\|f64)?", key)
         if m:
-            items.append(([int(m.group(i)) for i in (1, 2, 3)], int(m.group(4)), table[key]["plan"]))
+            items[m.group(7) is not None].append(([int(m.group(i)) for i in (1, 2, 3)], int(m.group(4)),
+                                                  table[key]["plan"]))
+        # a complex128 solve without a plan of its own runs the complex64 plan
+        if m and m.group(7) is None and key + "|f64" not in table:
+            items[True].append(items[False][-1])
     for item in filter(None, extra.split("|")):
         grid, plan = item.split("=", 1)
         n = [int(x) for x in grid.split("x")]
-        items.append((n[:3], n[3], plan))
+        items[False].append((n[:3], n[3], plan))
+        items[True].append((n[:3], n[3], plan))
     shapes = []
-    for f64 in (False, True):  # complex64, then the complex128 kernels of the same plans
-        for N, C, plan in items:
+    for f64 in (False, True):  # complex64, then the complex128 kernels
+        for N, C, plan in items[f64]:
             for s in launch_shapes(N, C, plan, f64):
                 if s not in shapes:
                     shapes.append(s)

@@ -1,8 +1,9 @@
 """FFT plan cache for the autotuner.
 
 A plan (per-axis Stockham radix order and workgroup tile width, see
-``wsim_solver_set_plan``) is keyed by grid size, components, right-hand sides
-and wiggle count.  Lookups consult the user cache (``$WSIM_PLAN_CACHE``,
+``wsim_solver_set_plan``) is keyed by grid size, components, right-hand sides,
+wiggle count and precision (complex128 solves, whose elements take twice the
+registers and LDS, have plans of their own).  Lookups consult the user cache (``$WSIM_PLAN_CACHE``,
 default ``~/.cache/wavesim_amd/plans.json``) and then the plans shipped with
 the package (``tuned_plans.json``, produced by ``MBSSolver.tune`` on MI355X).
 """
@@ -21,8 +22,8 @@ def cache_path() -> str:
                           os.path.join(os.path.expanduser("~"), ".cache", "wavesim_amd", "plans.json"))
 
 
-def key(N, nrhs: int, nw: int) -> str:
-    return "x".join(str(int(n)) for n in N) + f"|rhs{int(nrhs)}|nw{int(nw)}"
+def key(N, nrhs: int, nw: int, f64: bool = False) -> str:
+    return "x".join(str(int(n)) for n in N) + f"|rhs{int(nrhs)}|nw{int(nw)}" + ("|f64" if f64 else "")
 
 
 def _load(path: str) -> dict:
@@ -33,19 +34,22 @@ def _load(path: str) -> dict:
         return {}
 
 
-def lookup(N, nrhs: int, nw: int) -> Optional[str]:
-    k = key(N, nrhs, nw)
-    for path in (cache_path(), SHIPPED):
-        d = _load(path)
-        if k in d:
-            return d[k]["plan"] if isinstance(d[k], dict) else d[k]
+def lookup(N, nrhs: int, nw: int, f64: bool = False) -> Optional[str]:
+    """The plan for the shape; a complex128 solve without a plan of its own
+    takes the complex64 one (the library's shipped table does the same)."""
+    keys = [key(N, nrhs, nw, f64)] + ([key(N, nrhs, nw)] if f64 else [])
+    tables = [_load(cache_path()), _load(SHIPPED)]
+    for k in keys:
+        for d in tables:
+            if k in d:
+                return d[k]["plan"] if isinstance(d[k], dict) else d[k]
     return None
 
 
-def store(N, nrhs: int, nw: int, plan: str, ms: float) -> None:
+def store(N, nrhs: int, nw: int, plan: str, ms: float, f64: bool = False) -> None:
     path = cache_path()
     d = _load(path)
-    d[key(N, nrhs, nw)] = {"plan": plan, "ms_per_iteration": ms}
+    d[key(N, nrhs, nw, f64)] = {"plan": plan, "ms_per_iteration": ms}
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         with open(path, "w") as f:

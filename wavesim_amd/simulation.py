@@ -307,7 +307,7 @@ class WaveSimBase:
                                      complex(self.k02e), self.roi, self.energy_calculation_interval,
                                      nrhs=nrhs, dtype=self.cdtype)
             self._solver_nrhs = nrhs
-            plan = plans.lookup(self.N, nrhs, len(self.wiggles))
+            plan = plans.lookup(self.N, nrhs, len(self.wiggles), f64=not self.single)
             if plan:
                 try:
                     self._solver.plan = plan
@@ -339,7 +339,7 @@ class WaveSimBase:
         per axis) and remember it for later solvers of the same shape."""
         sv = self.solver(nrhs)
         plan, ms = sv.tune(reps)
-        plans.store(self.N, nrhs, len(self.wiggles), plan, ms)
+        plans.store(self.N, nrhs, len(self.wiggles), plan, ms, f64=not self.single)
         return plan, ms
 
     def _prepare(self, source: Source) -> Source:
