@@ -23,6 +23,8 @@ def test_pass_names_from_template_arguments():
                       "wsim::FixedPlan<320, 16, 320, 3, false, 20, 16> >(wsim::PassArgs const*)") \
         == "pass_mid_green[C=1,fixed]"
     assert short_name("wsim::control_kernel(wsim::DevState*, double const*, int)") == "control"
+    assert short_name("wsim::control_cycle_kernel(wsim::DevState*, double const*, int, int, long long)") \
+        == "control_cycle"
 
 
 def test_measured_traffic_matches_algorithmic_bytes():
@@ -84,3 +86,33 @@ def test_network_gemm_roles_follow_the_forward(monkeypatch):
               "projection Q1": (8, L ** 3, W), "projection Q2": (2, L ** 3, 8)}
     for role, shape in zip(roles, calls):
         assert shapes[role] == shape, role
+
+
+def _write_csv(path, header, rows):
+    import csv
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(header)
+        w.writerows(rows)
+
+
+def test_summaries_leave_out_launches_queued_after_a_stop(tmp_path):
+    """Pass launches that return at once (queued after the solve stopped) do
+    not dilute the per-pass averages of the trace, counter and traffic
+    summaries."""
+    from wavesim_amd import rocprof_summary as rs
+    k = "void wsim::axis_pass_kernel<true, 1, 0, 3, 1, 512, 16>(wsim::PassArgs const*)"
+    t0 = 0
+    rows = []
+    for dur in [300_000] * 6 + [5_000] * 2:
+        rows.append([k, t0, t0 + dur])
+        t0 += dur + 1000
+    _write_csv(str(tmp_path / "tr" / "x_kernel_trace.csv"), ["Kernel_Name", "Start_Timestamp", "End_Timestamp"], rows)
+    line = [l for l in rs.stats(str(tmp_path / "tr")).splitlines() if l.startswith("pass_inv_mix_fwd")][0].split()
+    assert line[1] == "6" and float(line[2]) == 300.0 and line[-1] == "2"
+    crow = [[k, "SQ_INSTS_VALU", v] for v in [4e6] * 6 + [10.0] * 2]
+    _write_csv(str(tmp_path / "sq" / "pmc_counter_collection.csv"), ["Kernel_Name", "Counter_Name", "Counter_Value"], crow)
+    table = rs.counters([str(tmp_path / "sq")], str(tmp_path / "tr"))
+    valu = [l for l in table.splitlines() if l.startswith("SQ_INSTS_VALU")][0].split()
+    assert float(valu[4]) == 4.0  # the fused pass's column: 4 M per busy dispatch

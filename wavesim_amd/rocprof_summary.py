@@ -46,6 +46,8 @@ def short_name(kernel: str) -> str:
         # plan-specialised kernels (FixedPlan<...>) are marked "fixed"
         tag = ",fixed" if "FixedPlan" in kernel else ""
         return _PASS.get(m.groups()[:4], "axis_pass") + f"[C={m.group(5)}{tag}]"
+    if "control_cycle_kernel" in kernel:
+        return "control_cycle"
     if "control_kernel" in kernel:
         return "control"
     if "copyBuffer" in kernel:
@@ -87,12 +89,20 @@ def counter_means(d: str, counter: str, skip_idle: bool = False):
 
 
 def stats(d: str) -> str:
+    """Per-kernel launch statistics.  Pass launches that returned at once (the
+    launches queued after a solve stopped: below 10% of the pass's median) are
+    counted in `idle` and left out of the averages."""
     dur = kernel_durations(d)
     total = sum(sum(v) for v in dur.values())
-    lines = [f"{'kernel':28s} {'calls':>6s} {'avg_us':>9s} {'median_us':>9s} {'min_us':>9s} {'time%':>6s}"]
+    lines = [f"{'kernel':28s} {'calls':>6s} {'avg_us':>9s} {'median_us':>9s} {'min_us':>9s} {'time%':>6s} {'idle':>5s}"]
     for k, v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
+        idle = 0
+        if "axis_pass_kernel" in k:
+            med = statistics.median(v)
+            busy = [x for x in v if x >= 0.1 * med]
+            idle, v = len(v) - len(busy), busy
         lines.append(f"{short_name(k):28s} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} "
-                     f"{statistics.median(v) / 1e3:9.2f} {min(v) / 1e3:9.2f} {100 * sum(v) / total:6.2f}")
+                     f"{statistics.median(v) / 1e3:9.2f} {min(v) / 1e3:9.2f} {100 * sum(v) / total:6.2f} {idle:5d}")
     return "\n".join(lines)
 
 
@@ -120,6 +130,42 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str) -> str:
     lines.append(f"# {'pass':16s} {'fetch_MB':>10s} {'write_MB':>10s} {'dispatches':>10s}")
     for name, (fb, wb, n) in rows.items():
         lines.append(f"# {name:16s} {fb / 1e6:10.1f} {wb / 1e6:10.1f} {n:10d}")
+    return "\n".join(lines)
+
+
+def counters(dirs, trace: str) -> str:
+    """Mean per dispatch (millions, summed over the device) of every counter
+    collected in `dirs` for the per-iteration passes, the kernel-trace average
+    duration from `trace`, and VALU busy = SQ_INSTS_VALU x 4 cycles (a wave64
+    VALU instruction on a 16-lane SIMD) / 1024 SIMDs over that duration at
+    2.4 GHz (an upper clock, so a lower bound on busy)."""
+    passes = ["pass_fwd", "pass_mid_green", "pass_inv", "pass_inv_mix_fwd"]
+    table = {}
+    for d in dirs:
+        acc = defaultdict(lambda: defaultdict(list))
+        with open(_one("*counter_collection.csv", d)) as f:
+            for r in csv.DictReader(f):
+                name = short_name(r["Kernel_Name"]).split("[")[0]
+                if name in passes:
+                    acc[r["Counter_Name"]][name].append(float(r["Counter_Value"]))
+        for c, per in acc.items():
+            # launches queued after a stop read next to nothing
+            table[c] = {n: sum(x for x in v if x >= 0.01 * max(v)) / max(1, sum(1 for x in v if x >= 0.01 * max(v)))
+                        for n, v in per.items()}
+    dur = {}
+    for k, v in kernel_durations(trace).items():
+        name = short_name(k).split("[")[0]
+        if name in passes:
+            med = statistics.median(v)
+            busy = [x for x in v if x >= 0.1 * med]
+            dur[name] = sum(busy) / len(busy) / 1e3
+    lines = [f"{'counter':34s}" + "".join(f"{p:>18s}" for p in passes)]
+    for c in sorted(table):
+        lines.append(f"{c:34s}" + "".join(f"{table[c].get(p, float('nan')) / 1e6:18.3f}" for p in passes))
+    lines.append(f"{'duration_us (kernel trace)':34s}" + "".join(f"{dur.get(p, float('nan')):18.2f}" for p in passes))
+    if "SQ_INSTS_VALU" in table:
+        busy = [table["SQ_INSTS_VALU"].get(p, 0) * 4 / 1024 / (dur.get(p, float("inf")) * 2400) for p in passes]
+        lines.append(f"{'valu_busy':34s}" + "".join(f"{b:18.2f}" for b in busy))
     return "\n".join(lines)
 
 
@@ -163,12 +209,17 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     for name in ("fetch", "write", "copy-fetch", "copy-write"):
         t.add_argument("--" + name, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--trace", required=True)
     n = sub.add_parser("fno")
     n.add_argument("dir")
     n.add_argument("--layers", type=int, default=4)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats(a.dir))
+    elif a.cmd == "counters":
+        print(counters(a.dirs, a.trace))
     elif a.cmd == "fno":
         print(fno_forward(a.dir, a.layers))
     else:
