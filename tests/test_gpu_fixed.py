@@ -91,3 +91,38 @@ def test_complex128_fixed_kernels_match_run_time_plan_kernels(gpu, monkeypatch):
     assert nf1 == 2 and nf0 == 0
     assert it1 == it0
     assert rel_l2(E1, E0) <= 1e-12
+
+
+def test_complex128_320_cubed_runs_its_own_plan(gpu, monkeypatch):
+    """The 320^3 workload in complex128 takes the complex128 plan shipped for
+    its shape (not the complex64 one), on plan-specialised double kernels; its
+    field equals the run-time-plan double kernels' on the default plan (another
+    factorisation, so double rounding over 360 iterations) to 1e-11, with the
+    same iterations."""
+    import json
+    import torch
+    from wavesim_amd import WaveSim, plans
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, source = WORKLOADS["disordered_3d_256"]()
+    opt = dict(opt, single_precision=False)
+    table = json.load(open(plans.SHIPPED))
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WSIM_FIXED", mode)
+        sim = WaveSim(nmap, opt)
+        sv = sim.solver(1)
+        want = table[plans.key(sim.N, 1, len(sim.wiggles), f64=True)]["plan"]
+        src = sim._prepare(source)
+        sv.set_source(0, src.positions, src.values)
+        out.append((sv.plan, sv.fixed_launches))
+        sv.solve(sim.max_iterations(), sim.energy_threshold)
+        torch.cuda.synchronize()
+        out[-1] += (sv.result(0).iterations, sv.field(0))
+        sv.close()
+    (p1, nf1, it1, E1), (p0, nf0, it0, E0) = out
+    print(f"complex128 320^3: plan {p1} ({nf1} fixed launches), {it1} / {it0} iterations "
+          f"(run-time-plan kernels on {p0}), rel-L2 {rel_l2(E1, E0):.2e}", flush=True)
+    assert p1 == want and want != table[plans.key(sim.N, 1, len(sim.wiggles))]["plan"]
+    assert nf1 == 4 and nf0 == 0 and p0 != want
+    assert E1.dtype == np.complex128 and it1 == it0
+    assert rel_l2(E1, E0) <= 1e-11
