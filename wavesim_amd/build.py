@@ -30,12 +30,12 @@ ARCH = os.environ.get("WSIM_OFFLOAD_ARCH", "gfx950")
 
 # (contig, mode, prologue, epilogue) -- must match WSIM_FOR_EACH_VARIANT
 VARIANTS = [(1, 0, 1, 0), (1, 1, 0, 1), (1, 1, 0, 3), (1, 1, 0, 2), (1, 2, 1, 1), (1, 2, 1, 2),
-            (1, 1, 0, 4), (1, 2, 1, 4), (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
+            (1, 1, 0, 4), (1, 1, 0, 5), (1, 2, 1, 4), (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0)]
 COMPONENTS = (1, 3)
 # kernel families: 0 complex64, 1 complex64 compact, 2 complex128 (the MBS
 # passes only -- must match WSIM_FOR_EACH_VARIANT_F64)
 FAMILY_F64 = 2
-PSTD_EPILOGUE = 4
+PSTD_EPILOGUES = (4, 5)  # the PSTD update, and the update fused with the next step's first pass
 
 
 def hipcc() -> str:
@@ -96,7 +96,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
              (os.path.join(CSRC, "problem.hip"), os.path.join(OBJ, "problem.o"), [])]
     for ct, m, p, e in VARIANTS:
         for c in COMPONENTS:
-            for k in (0, 1) if e == PSTD_EPILOGUE else (0, 1, FAMILY_F64):
+            for k in (0, 1) if e in PSTD_EPILOGUES else (0, 1, FAMILY_F64):
                 name = f"v_{ct}_{m}_{p}_{e}_{c}_{k}"
                 units.append((os.path.join(CSRC, "variant.hip"), os.path.join(OBJ, name + ".o"),
                               [f"-DWSIM_CONTIG={ct}", f"-DWSIM_MODE={m}", f"-DWSIM_PRO={p}",

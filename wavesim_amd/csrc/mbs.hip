@@ -804,7 +804,7 @@ struct Solver {
     L.args.twf = d_twf[axis];
     L.args.twi = d_twi[axis];
     int blocks = 0;
-    configure(L.args, contig, mode, axis, blocks, L.lds, epi == EPI_MIX_FWD);
+    configure(L.args, contig, mode, axis, blocks, L.lds, epi == EPI_MIX_FWD || epi == EPI_PSTD_FWD);
     // the compact register family whenever the tile fits it (WSIM_COMPACT=0 disables)
     const char* env_k = std::getenv("WSIM_COMPACT");
     const bool compact = !f64 && !(env_k && std::atoi(env_k) == 0) &&
@@ -843,11 +843,13 @@ struct Solver {
     // algorithmic bytes per voxel, averaged over an even and an odd
     // iteration (E' is read and written on even iterations only): the mix
     // passes read T, gamma, dE' (+ E') and write dE' (+ E') (+ the next T)
-    // (the PSTD update reads T, E, E_prev and c1..c3 and writes E_next)
+    // (the PSTD update reads T, E, E_prev and c1..c3 and writes E_next (+ the
+    // next T))
     const double perv = epi == EPI_MIX ? 32.0 * C + 8.0
                         : epi == EPI_MIX_FWD ? 40.0 * C + 8.0
                         : epi == EPI_PSTD ? 32.0 * C + 12.0
-                                          : 16.0 * C;
+                        : epi == EPI_PSTD_FWD ? 40.0 * C + 12.0
+                                              : 16.0 * C;
     L.bytes = perv * (double)nvox * nrhs * (f64 ? 2.0 : 1.0);
     if (L.lds > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)L.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -897,6 +899,15 @@ struct Solver {
   // update in the last pass
   void build_pstd_step(std::vector<Launch>& out) {
     const int D = (int)axes.size();
+    // 2-D: the update also runs the next step's first forward pass (same
+    // axis, same lines; that axis with at least two FFT stages), as the MBS
+    // iteration does, and the first pass runs once, before step 1 -- a whole
+    // latency-bound pass fewer per step (1152^2: 0.164 -> 0.131 s for 2006
+    // steps).  3-D grids keep them apart: the fused run-time-plan update
+    // holds too many registers (320^3: 474 us against 304 + 132 us).
+    // WSIM_FUSE=0 keeps them apart everywhere.
+    const char* env_f = std::getenv("WSIM_FUSE");
+    const bool fuse = D == 2 && tab[axes[0]].plan.nst >= 2 && !(env_f && std::atoi(env_f) == 0);
     if (D == 1) {
       out.push_back(make_launch(true, MODE_MID, PRO_SRC_RAMP, EPI_PSTD, axes[0], "pstd_step_1d"));
     } else {
@@ -919,8 +930,10 @@ struct Solver {
         L.args.out = T;
         out.push_back(L);
       }
-      Launch LE = make_launch(true, MODE_INV, PRO_NONE, EPI_PSTD, axes[0], "pstd_inv_update");
+      Launch LE = make_launch(true, MODE_INV, PRO_NONE, fuse ? EPI_PSTD_FWD : EPI_PSTD, axes[0],
+                              fuse ? "pstd_inv_update_fwd" : "pstd_inv_update");
       LE.args.in = T;
+      LE.args.out = T;
       out.push_back(LE);
     }
     for (auto& L : out) {
@@ -931,6 +944,10 @@ struct Solver {
       L.args.Sp = S;
       L.args.srcline = srcline;
       L.args.partials = partials;
+    }
+    if (fuse) {
+      pre = out.front();
+      out.erase(out.begin());
     }
   }
 

@@ -35,6 +35,7 @@ using KernelFn = void (*)(const PassArgs*);
   X(1, 2, 1, 1)                  \
   X(1, 2, 1, 2)                  \
   X(1, 1, 0, 4)                  \
+  X(1, 1, 0, 5)                  \
   X(1, 2, 1, 4)                  \
   X(0, 0, 0, 0)                  \
   X(0, 1, 0, 0)                  \

@@ -18,6 +18,8 @@
 //                 portion) times the next wiggle's ramps goes to LDS and
 //                 through the forward FFT stages to the work field.  The
 //                 first forward pass then runs only once, before iteration 1.
+//   EPI_PSTD_FWD  the PSTD update (EPI_PSTD below), then the next step's first
+//                 forward pass on the new time level, fused the same way.
 //
 // Layout: a field is [rhs][comp][z][x][y] complex64 (MATLAB column-major
 // [y,x,z,pol], y contiguous).  A pass along axis `a` sees each component as
@@ -164,7 +166,7 @@ struct DevState {
 
 enum { MODE_FWD = 0, MODE_INV = 1, MODE_MID = 2 };
 enum { PRO_NONE = 0, PRO_SRC_RAMP = 1 };
-enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2, EPI_MIX_FWD = 3, EPI_PSTD = 4 };
+enum { EPI_NONE = 0, EPI_MIX = 1, EPI_RAMP = 2, EPI_MIX_FWD = 3, EPI_PSTD = 4, EPI_PSTD_FWD = 5 };
 
 struct PassArgs {
   int N[3];  // Ny Nx Nz
@@ -748,8 +750,8 @@ struct Pass {
           if constexpr (K::kFixed) __builtin_amdgcn_sched_barrier(0);
           continue;
         }
-        if constexpr (E2 == EPI_PSTD) {
-          pstd_rows<R, ST>(v, it, j, s.nb, ci);
+        if constexpr (E2 == EPI_PSTD || E2 == EPI_PSTD_FWD) {
+          pstd_rows<R, ST>(v, it, t, j, s.nb, ci);
           continue;
         }
         const uint32_t g0 = gbase(cu, ci);
@@ -840,9 +842,12 @@ struct Pass {
   // PSTD update of the rows j + r*nb of one column (PSTD.m:108-116):
   // Etmp = lap E (+ A S on source lines); E_next = c2 E + c1 E_prev + c3 Etmp
   // into E_prev's buffer; energy partials of E_next - E exp(i dphi) in the ROI.
+  // EPI_PSTD_FWD: E_next times the next step's ramps also goes to LDS, the
+  // operand of the next step's first forward pass (as stage_load_global of
+  // that pass would form it).
   // Operands of a chunk of rows are loaded before any store (as mix_rows).
   template <int R, class ST>
-  __device__ __forceinline__ void pstd_rows(const V* v, int it, int j, int nb, const CI& ci) {
+  __device__ __forceinline__ void pstd_rows(const V* v, int it, int t, int j, int nb, const CI& ci) {
     static_assert(!F64, "the PSTD step is built for complex64 fields only");
     constexpr int ITS = its<R, ST>();
     constexpr int CH = 4 < R ? 4 : R;
@@ -879,6 +884,7 @@ struct Pass {
           const V en = mkv<V>((k2[q] * e.x + k1[q] * em.x) + k3[q] * et.x,
                                         (k2[q] * e.y + k1[q] * em.y) + k3[q] * et.y);
           stg(fprv_b[c], i, en);
+          if (EPI == EPI_PSTD_FWD) stl(lds, lidx(t, c, row), cmul(en, cmul(ldl(ramp2_a, row), ci.rc2)));
           if (in_roi && c >= a.roi_lo[3] && c <= a.roi_hi[3]) {
             const V d = csub(en, cmul(e, phase));
             eacc += (double)d.x * (double)d.x + (double)d.y * (double)d.y;
@@ -1010,7 +1016,7 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
   if (st->stopped[rhs]) return;
 
   // LDS: [CI x ncol][ramp_a x n][(ramp2 x n)][pe_a x n][twiddles][transform data]
-  constexpr bool FUSE = EPI == EPI_MIX_FWD;
+  constexpr bool FUSE = EPI == EPI_MIX_FWD || EPI == EPI_PSTD_FWD;
   CI* cols = reinterpret_cast<CI*>(smem);
   const size_t rn = ((size_t)a.n * sizeof(V) + 15) & ~(size_t)15;
   const size_t off_ramp = (a.ncol * sizeof(CI) + 15) & ~(size_t)15;
@@ -1055,7 +1061,8 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
   const long long rbase = (long long)rhs * a.rhs_stride + vblk;
   // PSTD: the first pass transforms the current time level
   const V* in_sel = as<V>((PRO == PRO_SRC_RAMP && a.pstd) ? a.fld[(it_cur - 1) & 1] : a.in);
-  if constexpr (EPI == EPI_PSTD) {
+  constexpr bool PSTD_STEP = EPI == EPI_PSTD || EPI == EPI_PSTD_FWD;
+  if constexpr (PSTD_STEP) {
     p.c1_b = wave_uniform(a.coef + vblk);
     p.c2_b = wave_uniform(a.coef + a.nvox + vblk);
     p.c3_b = wave_uniform(a.coef + 2 * a.nvox + vblk);
@@ -1065,7 +1072,7 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const long long cb = rbase + (long long)c * a.nvox;
-    if constexpr (EPI == EPI_PSTD) {
+    if constexpr (PSTD_STEP) {
       p.fcur_b[c] = wave_uniform(a.fld[(it_cur - 1) & 1] + cb);
       p.fprv_b[c] = wave_uniform(a.fld[it_cur & 1] + cb);
       p.Sp_b[c] = wave_uniform(a.Sp + cb);
@@ -1386,7 +1393,7 @@ __global__ void __launch_bounds__(NT, K::kWaves) axis_pass_kernel(const PassArgs
     }
   }
 
-  if (EPI == EPI_MIX || EPI == EPI_MIX_FWD || EPI == EPI_PSTD) {
+  if (EPI == EPI_MIX || EPI == EPI_MIX_FWD || EPI == EPI_PSTD || EPI == EPI_PSTD_FWD) {
     // block reduction of the ROI energy (fixed order: deterministic)
     __shared__ double red[NT / 64];
     double e = p.eacc;

@@ -36,6 +36,7 @@ _PASS = {  # (CONTIG, MODE, PRO, EPI) -> pass name
     ("true", "2", "1", "1"): "pass_mid_pro_epi",
     ("true", "2", "1", "2"): "pass_mid_pro_epi",
     ("true", "1", "0", "4"): "pstd_inv_update",
+    ("true", "1", "0", "5"): "pstd_inv_update_fwd",
     ("true", "2", "1", "4"): "pstd_step_1d",
 }
 
