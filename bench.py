@@ -290,7 +290,7 @@ def pstd_rate(workload: str, nsteps: int, reps: int):
     added so that the time stepping is damped), `nsteps` steps from rest
     (energy threshold 0: the cap ends the run), the stepping timed over `reps`
     runs after one warm-up.  Algorithmic bytes per step: 16 B/voxel per FFT pass, 44 for
-    the update pass (T, E, E_prev, c1..c3 in; E_next out); 2-D steps fuse the
+    the update pass (T, E, E_prev, c1..c3 in; E_next out); the steps fuse the
     update with the next step's first forward pass (+8 for T out, one FFT pass
     fewer; WSIM_FUSE=0 keeps them apart)."""
     import torch
@@ -308,7 +308,7 @@ def pstd_rate(workload: str, nsteps: int, reps: int):
     t = float(np.median(ts))
     nvox = int(np.prod(p.N[:3]))
     ndim = sum(1 for n in p.N[:3] if n > 1)
-    fused = ndim == 2 and os.environ.get("WSIM_FUSE", "1") != "0"
+    fused = ndim >= 2 and os.environ.get("WSIM_FUSE", "1") != "0"
     bpv = 16 * (2 * ndim - 3) + 52 if fused else 16 * (2 * ndim - 2) + 44
     p.close()
     return {"mode": "pstd", "workload": workload, "grid": p.N, "steps": its, "s": round(t, 6),

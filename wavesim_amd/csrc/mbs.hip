@@ -899,15 +899,14 @@ struct Solver {
   // update in the last pass
   void build_pstd_step(std::vector<Launch>& out) {
     const int D = (int)axes.size();
-    // 2-D: the update also runs the next step's first forward pass (same
-    // axis, same lines; that axis with at least two FFT stages), as the MBS
-    // iteration does, and the first pass runs once, before step 1 -- a whole
-    // latency-bound pass fewer per step (1152^2: 0.164 -> 0.131 s for 2006
-    // steps).  3-D grids keep them apart: the fused run-time-plan update
-    // holds too many registers (320^3: 474 us against 304 + 132 us).
-    // WSIM_FUSE=0 keeps them apart everywhere.
+    // the update also runs the next step's first forward pass (same axis,
+    // same lines; that axis with at least two FFT stages), as the MBS
+    // iteration does, and the first pass runs once, before step 1 (1152^2:
+    // 0.163 -> 0.128 s for 2006 steps; 320^3 on its shipped plan 0.701 ->
+    // 0.632 s for 928 steps, though 0.746 -> 0.773 s on the default plan,
+    // whose fused update holds more registers).  WSIM_FUSE=0 keeps them apart.
     const char* env_f = std::getenv("WSIM_FUSE");
-    const bool fuse = D == 2 && tab[axes[0]].plan.nst >= 2 && !(env_f && std::atoi(env_f) == 0);
+    const bool fuse = D >= 2 && tab[axes[0]].plan.nst >= 2 && !(env_f && std::atoi(env_f) == 0);
     if (D == 1) {
       out.push_back(make_launch(true, MODE_MID, PRO_SRC_RAMP, EPI_PSTD, axes[0], "pstd_step_1d"));
     } else {
@@ -1505,12 +1504,13 @@ struct Solver {
     dalloc(&p_dummy, 1);
     rebuild_launches();
     // the shipped plan of this shape and precision; a complex128 solver
-    // without one takes the complex64 plan
+    // without one takes the complex64 plan, and a PSTD solver (one wiggle)
+    // the MBS plan of its grid
     const ShippedPlan* shipped = nullptr;
     for (int want = f64 ? 1 : 0; want >= 0 && !shipped; --want)
       for (const ShippedPlan* sp = kShippedPlans; sp->plan; ++sp)
-        if (sp->n0 == N[0] && sp->n1 == N[1] && sp->n2 == N[2] && sp->c == C && sp->nrhs == nrhs && sp->nw == nw &&
-            sp->f64 == want) {
+        if (sp->n0 == N[0] && sp->n1 == N[1] && sp->n2 == N[2] && sp->c == C && sp->nrhs == nrhs &&
+            (sp->nw == nw || pstd) && sp->f64 == want) {
           shipped = sp;
           break;
         }
