@@ -49,6 +49,10 @@ int wsim_pstd_run(wsim_pstd* p, double max_iterations, double energy_threshold, 
  * factor exp(-i angle A_next), PSTD.m:125-126) and the step energies. */
 int wsim_pstd_result(wsim_pstd* p, wsim_cfloat* E_out, double* diff_energy, int64_t cap);
 int wsim_pstd_destroy(wsim_pstd* p);
+/* Launches of one step that run plan-specialised pass kernels (the grid's
+ * shipped plan compiled with its FFT plan as constants; no counterpart in the
+ * reference, whose PSTD.m calls fftn/ifftn): >= 0, or a negative error code. */
+int wsim_pstd_fixed_launches(wsim_pstd* p);
 
 #ifdef __cplusplus
 }

@@ -32,12 +32,14 @@ def test_320_cubed_shipped_plan_shapes():
 
 
 def test_1152_squared_two_dimensional_shapes():
-    """2-D: the first axis fused (and its once-per-solve first forward pass),
-    the second the middle pass (no x passes)."""
+    """2-D: the first axis fused (and its once-per-solve first forward pass;
+    the PSTD update fused alike, the same tile as the MBS one), the second the
+    middle pass (no x passes)."""
     shapes = fp.launch_shapes([1152, 1152, 1], 1, "y:8,3,3,16/24/512/noxcd;x:16,3,3,8/16/0/xcd")
-    assert [s[:4] for s in shapes] == [fp.FUSED_FIRST, fp.FIRST_FWD, fp.MID_STRIDED]
+    assert [s[:4] for s in shapes] == [fp.FUSED_FIRST, fp.FIRST_FWD, fp.PSTD_FUSED, fp.MID_STRIDED]
+    assert shapes[2][4:] == shapes[0][4:]
     # strided 1152-point tiles: 4 columns (1152 x 9 x 8 B = 82 944 B > 80 KB for 8)
-    assert shapes[2][7] == 4
+    assert shapes[3][7] == 4
 
 
 def test_single_stage_first_axis_uses_separate_mix_pass():

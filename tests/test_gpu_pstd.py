@@ -127,3 +127,28 @@ def test_pstd_iteration_cap_and_reuse(gpu):
     assert np.array_equal(E1, E2)
     assert rel_l2(E1, Eo) <= TOL
     g.close()
+
+
+@pytest.mark.gpu
+def test_pstd_on_a_shipped_grid_runs_specialised_kernels(gpu, monkeypatch):
+    """A PSTD solver on a grid with a shipped MBS plan (the 128^3 spheres
+    medium, 192^3 grid) runs that plan, its x and z passes and its fused
+    update on plan-specialised kernels: the same steps and the field of the
+    run-time-plan kernels on the same plan to 1e-5."""
+    from wavesim_amd import PSTD
+    from wavesim_amd.workloads import WORKLOADS
+    nmap, opt, src = WORKLOADS["spheres_3d_128"]()
+    popt = dict(opt, boundary_type="ARL", energy_threshold=0.0)
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WSIM_FIXED", mode)
+        p = PSTD(np.asarray(nmap) + 0.02j, popt)
+        nf = p.fixed_launches
+        E, st = p.exec(src, max_iterations=120)
+        out.append((nf, st.iterations, E))
+        p.close()
+    (nf1, it1, E1), (nf0, it0, E0) = out
+    print(f"PSTD 192^3: fixed launches {nf1}, steps {it1} / {it0}, rel-L2 {rel_l2(E1, E0):.2e}", flush=True)
+    assert nf1 == 4 and nf0 == 0
+    assert it1 == it0 > 0
+    assert rel_l2(E1, E0) <= 1e-5

@@ -128,6 +128,7 @@ SYMBOLS = {
     "wsim_pstd_run": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "wsim_pstd_result": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_int64]),
     "wsim_pstd_destroy": (C.c_int, [C.c_void_p]),
+    "wsim_pstd_fixed_launches": (C.c_int, [C.c_void_p]),
     "wsim_last_error": (C.c_char_p, []),
     "wsim_abi_version": (C.c_int, []),
     "wsim_selftest": (C.c_int, []),

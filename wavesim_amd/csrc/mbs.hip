@@ -2672,6 +2672,13 @@ int wsim_pstd_result(wsim_pstd* p, wsim_cfloat* E_out, double* diff_energy, int6
   });
 }
 
+int wsim_pstd_fixed_launches(wsim_pstd* p) {
+  return guard([&]() -> int {
+    require(p != nullptr, "null handle");
+    return p->s.fixed_launches();
+  });
+}
+
 int wsim_pstd_destroy(wsim_pstd* p) {
   return guard([&]() -> int {
     delete p;

@@ -29,6 +29,7 @@ MAX_ITS = 6
 # pass variants as (contig, mode, prologue, epilogue): mode 0 fwd, 1 inv, 2 mid
 FWD_STRIDED, INV_STRIDED, MID_STRIDED, FUSED_FIRST, MIX_FIRST = (0, 0, 0, 0), (0, 1, 0, 0), (0, 2, 0, 0), (1, 1, 0, 3), (1, 1, 0, 1)
 FIRST_FWD = (1, 0, 1, 0)  # the first forward pass (source, ramps): once per solve when the last pass is fused
+PSTD_FUSED = (1, 1, 0, 5)  # the PSTD update fused with the next step's first forward pass (complex64 scalar)
 
 
 def ept_for(C: int, family: int) -> int:
@@ -155,7 +156,7 @@ def launch_shapes(N, C, plan_text, f64=False):
     def add(variant, axis):
         rad, ncap, thr = plan[axis]
         contig, mode, pro, epi = variant
-        cfg = configure(N, C, rad, ncap, thr, axis, bool(contig), mode, epi == 3, f64)
+        cfg = configure(N, C, rad, ncap, thr, axis, bool(contig), mode, epi in (3, 5), f64)
         if cfg is None:
             return
         ncol, nthr, family, waves, early = cfg
@@ -167,6 +168,10 @@ def launch_shapes(N, C, plan_text, f64=False):
     first = axes[0]
     add(FUSED_FIRST if len(plan[first][0]) >= 2 else MIX_FIRST, first)
     add(FIRST_FWD, first)
+    # a PSTD solver of the grid runs the same plan: its fused update (the other
+    # PSTD passes are the MBS shapes)
+    if C == 1 and not f64 and len(plan[first][0]) >= 2:
+        add(PSTD_FUSED, first)
     for a in axes[1:-1]:
         add(FWD_STRIDED, a)
         add(INV_STRIDED, a)

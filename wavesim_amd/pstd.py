@@ -104,6 +104,13 @@ class PSTD:
             self._h = h
         return self._h
 
+    @property
+    def fixed_launches(self) -> int:
+        """Launches of one step that run plan-specialised pass kernels."""
+        n = capi.lib().wsim_pstd_fixed_launches(self._handle())
+        capi.check(n if n < 0 else 0)
+        return n
+
     def close(self):
         if self._h is not None:
             capi.lib().wsim_pstd_destroy(self._h)
