@@ -1,6 +1,6 @@
 """HIP path vs the CPU oracle (complex64 restatement of the reference loop).
 
-Bar: relative L2 <= 1e-5 in complex64 (north star), with the same iteration
+Bar: relative L2 <= 1e-5 in complex64, with the same iteration
 count; the every-iteration ROI energies agree to 1e-4 relative.
 """
 from __future__ import annotations
