@@ -116,3 +116,29 @@ def test_summaries_leave_out_launches_queued_after_a_stop(tmp_path):
     table = rs.counters([str(tmp_path / "sq")], str(tmp_path / "tr"))
     valu = [l for l in table.splitlines() if l.startswith("SQ_INSTS_VALU")][0].split()
     assert float(valu[4]) == 4.0  # the fused pass's column: 4 M per busy dispatch
+
+
+def test_network_forward_summary_labels_the_last_forward(tmp_path):
+    """The per-role table of a network forward reads the GEMM launches that
+    follow the last forward's first launch (the mixing-weight assembly)."""
+    from wavesim_amd import rocprof_summary as rs
+    from wavesim_amd.learned.fno import Config, forward_gemm_roles
+    roles = forward_gemm_roles(Config(layers=4))
+    rows, t = [], 0
+    for fwd in range(2):
+        rows.append(["wfno::complex_weights_kernel(float const*)", t, t + 5000])
+        t += 6000
+        for k, role in enumerate(roles):
+            dur = 10_000 if role == "complex mixing" else 20_000 + 1000 * fwd
+            rows.append(["wfno::gemm_kernel(wfno_gemm_args)", t, t + dur])
+            t += dur + 100
+            if k % 10 == 0:
+                rows.append(["wfno::bias_act_kernel(float*)", t, t + 3000])
+                t += 3100
+    _write_csv(str(tmp_path / "fno" / "x_kernel_trace.csv"), ["Kernel_Name", "Start_Timestamp", "End_Timestamp"],
+               rows)
+    out = rs.fno_forward(str(tmp_path / "fno"))
+    table = {l.split()[0] + " " + l.split()[1]: l.split() for l in out.splitlines()[2:]}
+    assert float(table["complex mixing"][3]) == 10.0 and table["complex mixing"][2] == "12"
+    assert float(table["pointwise Wp"][3]) == 21.0  # the second (last) forward's durations
+    assert f"{len(roles)} GEMM launches" in out.splitlines()[0]
