@@ -152,3 +152,25 @@ def test_pstd_on_a_shipped_grid_runs_specialised_kernels(gpu, monkeypatch):
     assert nf1 == 4 and nf0 == 0
     assert it1 == it0 > 0
     assert rel_l2(E1, E0) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("builder", [_disordered_2d, _disordered_3d])
+def test_pstd_fused_update_equals_separate_passes(gpu, builder, monkeypatch):
+    """The update fused with the next step's first forward pass (the
+    default) and the two as separate launches (WSIM_FUSE=0) compute the same
+    steps: same step count and stop, fields equal to rounding."""
+    import wavesim_amd as w
+    n, opt, src = builder()
+    gs, _ = _sources(src)
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("WSIM_FUSE", mode)
+        g = w.PSTD(n, opt)
+        E, st = g.exec(gs)
+        out.append((E, st))
+        g.close()
+    (E1, s1), (E0, s0) = out
+    print(f"{builder.__name__}: steps {s1.iterations} / {s0.iterations}, rel-L2 {rel_l2(E1, E0):.2e}", flush=True)
+    assert s1.iterations == s0.iterations and s1.converged == s0.converged
+    assert rel_l2(E1, E0) <= 1e-6
